@@ -1,0 +1,343 @@
+"""Throughput benchmark of the escape-time hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload deep|fp64]
+
+Workloads (inputs seeded, tests/golden/configs.json):
+  deep  3840x2160 Mandelbrot perturbation at width 1e-1000, max_iter 200000,
+        centre on the main-cardioid boundary (1100 digits), chained BLA
+        (16 levels, tol 1e-6), Pauldelbrot glitch test.  Timed: the FP32-
+        mantissa FloatExp path; the FP64-mantissa FloatExp path (the
+        reference's FloatExp design) is timed on the same orbit and table.
+  fp64  Plain FP64 Mandelbrot at (-0.65, 0), width 4, max_iter 1000,
+        3840x2160 (config 1's view at 4K).
+A step is one frame of device work with the inputs resident in HBM: the
+BLA table build (deep) and the pixel pass with the palette colouring fused.
+The host orbit is computed once before timing.
+
+Prints one JSON line.  value = Giter/s, an iteration being one effective
+escape-time iteration of one sample (BLA-skipped iterations count,
+cardioid-shortcut pixels count 0).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+from decimal import Decimal
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X vector FP64
+FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32 (published; packed forms)
+HBM_PEAK_GBS = 8000.0
+# Algorithmic flops per executed step (counted from the reference formulas):
+FLOPS_FP64_ITER = 8        # z^2 + c: 4 mul + 4 add
+FLOPS_FEXP_PLAIN = 21      # 2 Z dz (8) + dz^2 2^s (9) + dc 2^-e (4)
+FLOPS_BLA_STEP = 14        # A dz + B dc: 8 mul + 6 add
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def load_configs():
+    with open(os.path.join(ROOT, "tests", "golden", "configs.json")) as f:
+        return json.load(f)
+
+
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+class Timer:
+    """HIP events on torch's current stream, which is the stream the library
+    launches on."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.pairs = []
+
+    def start(self):
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record()
+        return e
+
+    def stop(self, s):
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record()
+        self.pairs.append((s, e))
+
+    def mean_ms(self):
+        self.torch.cuda.synchronize()
+        if not self.pairs:
+            return float("nan")
+        return float(np.mean([a.elapsed_time(b) for a, b in self.pairs]))
+
+
+def run_deep(args, torch):
+    from fractura_amd import _abi, engine
+
+    cfg = load_configs()["config3"]
+    W, H = cfg["size"]
+    if args.size:
+        W, H = args.size
+    max_iter = cfg["max_iter"]
+    t0 = time.time()
+    plan32 = engine.plan_view(W, H, max_iter, Decimal(cfg["center_x"]), Decimal(cfg["center_y"]),
+                              Decimal(cfg["width"]), bbsa_order=32, bbsa_tol=1e-6,
+                              glitch_mode="Pauldelbrot", fexp_mantissa=32)
+    log(f"orbit: ref_len={plan32.view.ref_len} E={plan32.view.e_scale} ({time.time() - t0:.1f}s host)")
+    plan64 = engine.plan_view(W, H, max_iter, Decimal(cfg["center_x"]), Decimal(cfg["center_y"]),
+                              Decimal(cfg["width"]), bbsa_order=32, bbsa_tol=1e-6,
+                              glitch_mode="Pauldelbrot", fexp_mantissa=64)
+    assert plan32.engine == "fexp32" and plan64.engine == "fexp64"
+    dev = engine.get_device(0)
+    dev.set_stream(None)
+    orbit = engine._Orbit(dev, plan32, 0)
+    lut = engine.get_gpu_cmap_lut("inferno", 2048, False, dev.torch_device)
+    n = W * H
+    outs = {}
+    for name in ("fexp32", "fexp64"):
+        outs[name] = dict(
+            counts=torch.empty((H, W), dtype=torch.int32, device=dev.torch_device),
+            smooth=torch.empty((H, W), dtype=torch.float64, device=dev.torch_device),
+            iter_map=torch.empty((H, W), dtype=torch.float32, device=dev.torch_device),
+            glitch=torch.empty((H, W), dtype=torch.uint8, device=dev.torch_device),
+            rgb=torch.empty((H, W, 3), dtype=torch.uint8, device=dev.torch_device),
+            stats=torch.zeros(4, dtype=torch.int64, device=dev.torch_device))
+
+    def step(name, plan, timer=None, with_stats=False):
+        v = plan.view.copy()
+        dev.call("ap_bla_build", C.byref(orbit.abi), v.fractal_type, v.gen_n, v.gen_kr, v.gen_ki,
+                 plan.bla_tol, orbit.levels, engine.ptr(orbit.table))
+        o = outs[name]
+        ao = engine._outs(o["counts"], o["smooth"], o["iter_map"], o["glitch"], o["rgb"], lut,
+                          stats=o["stats"] if with_stats else None)
+        s = timer.start() if timer else None
+        engine._launch(dev, name, v, orbit, ao)
+        if timer:
+            timer.stop(s)
+
+    results = {}
+    for name, plan in (("fexp32", plan32), ("fexp64", plan64)):
+        outs[name]["stats"].zero_()
+        step(name, plan, with_stats=True)
+        torch.cuda.synchronize()
+        st = outs[name]["stats"].cpu().numpy().view(np.uint64)
+        iters, plain, bla = int(st[0]), int(st[1]), int(st[2])
+        for _ in range(args.warmup):
+            step(name, plan)
+        torch.cuda.synchronize()
+        timer = Timer(torch)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(name, plan, timer)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        kern_ms = timer.mean_ms()
+        flops = plain * FLOPS_FEXP_PLAIN + bla * FLOPS_BLA_STEP
+        results[name] = dict(iters=iters, plain=plain, bla=bla, wall_ms=wall * 1e3,
+                             kernel_ms=kern_ms, giters=iters / wall / 1e9,
+                             tflops=flops / (kern_ms * 1e-3) / 1e12 if kern_ms > 0 else 0.0)
+        log(f"{name}: {results[name]}")
+
+    c32 = outs["fexp32"]["counts"].cpu().numpy()
+    c64 = outs["fexp64"]["counts"].cpu().numpy()
+    s32 = outs["fexp32"]["smooth"].cpu().numpy()
+    s64 = outs["fexp64"]["smooth"].cpu().numpy()
+    agree = float(np.mean(c32 == c64))
+    esc = (s64 >= 0) & (s32 >= 0)
+    smooth_err = float(np.max(np.abs(s32[esc] - s64[esc]), initial=0.0))
+    log(f"fexp32 vs fexp64: count agreement {agree:.6f}, max smooth diff {smooth_err:.3g}, "
+        f"interior {np.mean(c64 >= max_iter):.4f}")
+
+    # Oracle subsample: a 48x48 block of the same view on the CPU restatement.
+    par = oracle_block(plan64, orbit_host=plan64.orbit, bla_tol=plan64.bla_tol, c64=c64, c32=c32)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_fexp(plan64, args.cpu_seconds)
+    r32, r64 = results["fexp32"], results["fexp64"]
+    return {
+        "metric": "Giter/s per GPU, FP32+FloatExp perturb vs FP64 at matched count accuracy",
+        "value": r32["giters"],
+        "unit": "Giter/s",
+        "ms_per_step": r32["wall_ms"],
+        "dtype": "fp32+i32exp",
+        "config": {"workload": "config3: Mandelbrot perturbation, chained BLA d=0..15, "
+                               "width 1e-1000, cardioid-boundary centre (1100 digits)",
+                   "size": [W, H], "max_iter": max_iter, "bla_levels": 16, "bla_tol": 1e-6,
+                   "glitch": "Pauldelbrot 1e-3", "ref_len": plan32.view.ref_len,
+                   "e_scale": plan32.view.e_scale},
+        "fp64_mantissa_value": r64["giters"],
+        "fp32_vs_fp64": r32["giters"] / r64["giters"] if r64["giters"] else None,
+        "count_agreement_fp32_vs_fp64": agree,
+        "max_smooth_diff_fp32_vs_fp64": smooth_err,
+        "oracle_subsample": par,
+        "iterations_per_step": r32["iters"],
+        "executed_steps": {"plain": r32["plain"], "bla": r32["bla"]},
+        "roofline": {"bound": "valu", "kernel": "tile_pass<EngFexp32<0>>",
+                     "achieved": r32["tflops"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": r32["tflops"] / FP32_PEAK_TFLOPS, "traffic": None,
+                     "kernel_ms": r32["kernel_ms"]},
+        "roofline_fp64": {"kernel": "tile_pass<EngFexp64<0>>", "achieved": r64["tflops"],
+                          "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                          "frac": r64["tflops"] / FP64_PEAK_TFLOPS, "kernel_ms": r64["kernel_ms"]},
+        "cpu_baseline": cpu,
+    }
+
+
+def oracle_block(plan, orbit_host, bla_tol, c64, c32, size=48):
+    """Counts of a centred size x size block on the CPU oracle (FP64-mantissa
+    FloatExp restatement) against both GPU paths."""
+    from oracle import oracle
+    v = plan.view.copy()
+    W, H = v.width, v.height
+    x0, y0 = (W - size) // 2, (H - size) // 2
+    sub = v.copy()
+    sub.width, sub.height = size, size
+    sub.ref_px = v.ref_px - x0 * v.factor
+    sub.ref_py = v.ref_py - y0 * v.factor
+    re, im, ref_len = orbit_host
+    bla = oracle.bla_build(re, im, ref_len, v.fractal_type, bla_tol, 16)
+    m = oracle.render("fexp64", sub, orbit=(re, im), bla=bla, levels=16)
+    blk64 = c64[y0:y0 + size, x0:x0 + size]
+    blk32 = c32[y0:y0 + size, x0:x0 + size]
+    return {"pixels": size * size, "fexp64_exact": float(np.mean(blk64 == m.counts)),
+            "fexp32_exact": float(np.mean(blk32 == m.counts))}
+
+
+def cpu_baseline_fexp(plan, seconds):
+    """Oracle FloatExp (FP64 mantissa) over a centred tile on all host cores,
+    tile side grown until the run takes about `seconds`."""
+    from oracle import oracle
+
+    v = plan.view.copy()
+    re, im, ref_len = plan.orbit
+    bla = oracle.bla_build(re, im, ref_len, v.fractal_type, plan.bla_tol, 16)
+    side = 32
+    model, cores = cpu_info()
+    threads = oracle.load().or_num_threads()
+    while True:
+        sub = v.copy()
+        side = min(side, v.width, v.height)
+        x0, y0 = (v.width - side) // 2, (v.height - side) // 2
+        sub.width = sub.height = side
+        sub.ref_px, sub.ref_py = v.ref_px - x0 * v.factor, v.ref_py - y0 * v.factor
+        t0 = time.perf_counter()
+        m = oracle.render("fexp64", sub, orbit=(re, im), bla=bla, levels=16)
+        dt = time.perf_counter() - t0
+        if dt > seconds / 4 or side >= min(v.width, v.height):
+            break
+        side *= 2
+    return {"value": m.iterations / dt / 1e9, "unit": "Giter/s", "cores": threads,
+            "kind": "port", "cpu": model,
+            "sample": f"centred {side}x{side} tile of the same view, FP64-mantissa FloatExp "
+                      f"restatement (oracle/apeiron_oracle.cpp, -O2 -ffp-contract=off, OpenMP "
+                      f"{threads} threads), {dt:.2f}s"}
+
+
+def run_fp64(args, torch):
+    from fractura_amd import engine
+
+    W, H = args.size or (3840, 2160)
+    plan = engine.plan_view(W, H, 1000, Decimal("-0.65"), Decimal("0.0"), Decimal("4.0"))
+    dev = engine.get_device(0)
+    dev.set_stream(None)
+    lut = engine.get_gpu_cmap_lut("inferno", 2048, False, dev.torch_device)
+    counts = torch.empty((H, W), dtype=torch.int32, device=dev.torch_device)
+    smooth = torch.empty((H, W), dtype=torch.float64, device=dev.torch_device)
+    it = torch.empty((H, W), dtype=torch.float32, device=dev.torch_device)
+    rgb = torch.empty((H, W, 3), dtype=torch.uint8, device=dev.torch_device)
+    stats = torch.zeros(4, dtype=torch.int64, device=dev.torch_device)
+
+    def step(timer=None, with_stats=False):
+        o = engine._outs(counts, smooth, it, None, rgb, lut, stats=stats if with_stats else None)
+        s = timer.start() if timer else None
+        engine._launch(dev, "fp64", plan.view, None, o)
+        if timer:
+            timer.stop(s)
+
+    step(with_stats=True)
+    torch.cuda.synchronize()
+    iters = int(stats.cpu().numpy().view(np.uint64)[0])
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    timer = Timer(torch)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(timer)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    kms = timer.mean_ms()
+    tflops = iters * FLOPS_FP64_ITER / (kms * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+        model, _ = cpu_info()
+        threads = oracle.load().or_num_threads()
+        t0 = time.perf_counter()
+        m = oracle.render("fp64", plan.view)
+        dt = time.perf_counter() - t0
+        same = bool(np.array_equal(m.counts, counts.cpu().numpy()))
+        cpu = {"value": m.iterations / dt / 1e9, "unit": "Giter/s", "cores": threads, "kind": "port",
+               "cpu": model, "sample": f"full {W}x{H} frame, oracle FP64 restatement, {dt:.2f}s; "
+                                       f"counts bit-exact vs GPU: {same}"}
+    return {
+        "metric": "Giter/s per GPU, FP32+FloatExp perturb vs FP64 at matched count accuracy",
+        "value": iters / wall / 1e9, "unit": "Giter/s", "ms_per_step": wall * 1e3, "dtype": "f64",
+        "config": {"workload": "plain FP64 Mandelbrot (-0.65,0) w=4 max_iter 1000", "size": [W, H]},
+        "iterations_per_step": iters,
+        "roofline": {"bound": "valu", "kernel": "tile_pass<EngFP64<0>>", "achieved": tflops,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS,
+                     "traffic": None, "kernel_ms": kms},
+        "cpu_baseline": cpu,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["deep", "fp64"], default="deep")
+    ap.add_argument("--size", type=int, nargs=2, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no HIP device"}))
+        sys.exit(1)
+    res = run_deep(args, torch) if args.workload == "deep" else run_fp64(args, torch)
+    line = {"metric": res.pop("metric"), "value": res.pop("value"), "unit": res.pop("unit"),
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": res.pop("ms_per_step"), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": res.pop("dtype"), "data": "synthetic",
+            "config": res.pop("config")}
+    line.update(res)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,134 @@
+"""ctypes view of the C ABI in include/apeiron_hip.h.
+
+The structures mirror the header field for field; `load()` opens the in-tree
+libapeiron_hip.so and declares every exported entry point.  There is no
+fallback: if the library is missing, loading raises.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libapeiron_hip.so")
+
+AP_OK = 0
+AP_ERR_ARG = -1
+AP_ERR_HIP = -2
+AP_ERR_UNSUPPORTED = -3
+
+AP_MODE_CENTRE = 0
+AP_MODE_ALL_SUBSAMPLES = 1
+AP_MODE_EDGE_SUBSAMPLES = 2
+
+AP_STAT_ITERATIONS = 0
+AP_STAT_PLAIN_STEPS = 1
+AP_STAT_BLA_STEPS = 2
+AP_STAT_SAMPLES = 3
+AP_STAT_COUNT = 4
+
+
+class ApView(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("factor", C.c_int32),
+        ("max_iter", C.c_int32), ("fractal_type", C.c_int32), ("gen_n", C.c_int32),
+        ("mode", C.c_int32), ("rebase_pass", C.c_int32),
+        ("gen_kr", C.c_double), ("gen_ki", C.c_double),
+        ("julia_cx", C.c_double), ("julia_cy", C.c_double),
+        ("julia_cx_lo", C.c_double), ("julia_cy_lo", C.c_double),
+        ("x_min", C.c_double), ("y_max", C.c_double),
+        ("x_min_lo", C.c_double), ("y_max_lo", C.c_double),
+        ("dx", C.c_double), ("dy", C.c_double),
+        ("dx_lo", C.c_double), ("dy_lo", C.c_double),
+        ("inv_ln2", C.c_double), ("ln_bailout_log", C.c_double),
+        ("ref_px", C.c_double), ("ref_py", C.c_double),
+        ("ref_cx", C.c_double), ("ref_cy", C.c_double),
+        ("ref_len", C.c_int32), ("e_scale", C.c_int32),
+        ("strict_glitch", C.c_int32), ("bla_levels", C.c_int32),
+        ("scheme_id", C.c_int32), ("is_hdr", C.c_int32),
+        ("palette_offset", C.c_double), ("color_density", C.c_double),
+        ("color_contrast", C.c_double),
+    ]
+
+    def copy(self):
+        v = ApView()
+        C.pointer(v)[0] = self
+        return v
+
+
+class ApOutputs(C.Structure):
+    _fields_ = [
+        ("counts", C.c_void_p), ("smooth", C.c_void_p), ("iter_map", C.c_void_p),
+        ("glitch", C.c_void_p), ("rgb", C.c_void_p), ("lut", C.c_void_p),
+        ("mask", C.c_void_p), ("cdf", C.c_void_p), ("num_bins", C.c_int32),
+        ("reserved0", C.c_int32), ("queue", C.c_void_p), ("queue_count", C.c_void_p),
+        ("stats", C.c_void_p),
+    ]
+
+
+class ApOrbit(C.Structure):
+    _fields_ = [("re", C.c_void_p), ("im", C.c_void_p), ("len", C.c_int32), ("reserved0", C.c_int32)]
+
+
+_P = C.c_void_p
+_I = C.c_int32
+_D = C.c_double
+_VIEW = C.POINTER(ApView)
+_OUT = C.POINTER(ApOutputs)
+_ORB = C.POINTER(ApOrbit)
+
+# name -> (restype, argtypes); every symbol the header declares.
+SIGNATURES = {
+    "ap_init": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "ap_destroy": (C.c_int, [_P]),
+    "ap_last_error": (C.c_char_p, [_P]),
+    "ap_set_stream": (C.c_int, [_P, _P]),
+    "ap_sync": (C.c_int, [_P]),
+    "ap_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "ap_free": (C.c_int, [_P, _P]),
+    "ap_memcpy_h2d": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "ap_memcpy_d2h": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "ap_memset": (C.c_int, [_P, _P, C.c_int, C.c_size_t]),
+    "ap_device_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
+    "ap_render_fp64": (C.c_int, [_P, _VIEW, _OUT]),
+    "ap_render_dd": (C.c_int, [_P, _VIEW, _OUT]),
+    "ap_render_pert64": (C.c_int, [_P, _VIEW, _ORB, _P, _OUT]),
+    "ap_render_fexp64": (C.c_int, [_P, _VIEW, _ORB, _P, _OUT]),
+    "ap_render_fexp32": (C.c_int, [_P, _VIEW, _ORB, _P, _OUT]),
+    "ap_bla_bytes": (C.c_size_t, [_I, _I]),
+    "ap_bla_build": (C.c_int, [_P, _ORB, _I, _I, _D, _D, _D, _I, _P]),
+    "ap_bla_export": (C.c_int, [_P, _P, _I, _I, _P]),
+    "ap_edge_mask": (C.c_int, [_P, _P, _I, _I, C.c_float, _P]),
+    "ap_compact": (C.c_int, [_P, _P, _I, _P, _P]),
+    "ap_glitch_bins": (C.c_int, [_P, _P, _I, _I, _P]),
+    "ap_hist_build": (C.c_int, [_P, _P, _I, _I, _I, _P]),
+    "ap_hist_cdf": (C.c_int, [_P, _P, _I, _P]),
+    "ap_colorize": (C.c_int, [_P, _VIEW, _P, _P, _I, _P, _P]),
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Open libapeiron_hip.so and declare its entry points (no fallback)."""
+    global _lib
+    if _lib is not None and path == LIB_PATH:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it with `python -m fractura_amd.build` "
+            "(the HIP path has no CPU fallback)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path == LIB_PATH:
+        _lib = lib
+    return lib
+
+
+class ApError(RuntimeError):
+    """A C-ABI call returned a negative status."""
+
+    def __init__(self, status, where, text):
+        super().__init__(f"{where} failed ({status}): {text}")
+        self.status = status

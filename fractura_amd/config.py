@@ -1,0 +1,157 @@
+"""Render parameters and the constants/formulas the escape-time path reads.
+
+Restates the path-relevant parts of the reference config.py: the smooth
+constants (config.py:25-26), the precision-bit and FloatExp-scale formulas
+(config.py:36-60), the option tables (config.py:78-150) and the
+RenderParameters dataclass (config.py:196-243).  Decimal precision is set per
+call with a local context instead of the reference's global one.
+"""
+import decimal
+from dataclasses import asdict, dataclass
+from decimal import Decimal
+
+# ln of 0.5*ln(65536) and 1/ln 2 as float64, as the reference passes them
+# (config.py:25-26).
+INV_LN2_F32 = 1.4426950408889634
+LN_BAILOUT_LOG = 1.7129177284488854
+
+DEFAULT_DECIMAL_PREC = 500
+GLOBAL_MAX_ITER = 500_000_000
+
+FRACTAL_NAMES = [
+    "Mandelbrot",
+    "Burning Ship",
+    "Julia Set (Mandelbrot)",
+    "Julia Set (Burning Ship)",
+    "General Mandelbrot",
+]
+
+FRACTAL_DEFAULTS = {
+    0: (Decimal("-0.65"), Decimal("0.0"), Decimal("4.0")),
+    1: (Decimal("-0.45"), Decimal("0.5"), Decimal("3.5")),
+    2: (Decimal("0.0"), Decimal("0.0"), Decimal("3.5")),
+    3: (Decimal("0.0"), Decimal("0.0"), Decimal("4.0")),
+    4: (Decimal("0.0"), Decimal("0.0"), Decimal("4.0")),
+}
+
+COLOR_SCHEMES = {
+    "Sqrt Fold": 0, "Logarithmic": 1, "Sigmoidal Tanh": 2, "Exponential Flare": 3,
+    "High-Frequency Ripples": 4, "Golden Ratio Dual Sine": 5, "Cosine Smooth": 6,
+    "Harmonic Cascade": 7, "Quintic S-Curve": 8, "Chirp Sweep": 9, "Soft Damping": 10,
+    "Histogram Equalized": 11,
+}
+
+PRECISION_OPTIONS = {
+    "1e-12 (FP64)": Decimal("1e-12"),
+    "1e-24 (DD)": Decimal("1e-24"),
+    "1e-300 (Perturbation)": Decimal("1e-300"),
+    "1e-10000+ (Floatexp)": Decimal("1e-646000000"),
+}
+
+BBSA_OPTIONS = {
+    "Off": {"order": 0, "tol": 0.0},
+    "4th-order": {"order": 4, "tol": 1e-4},
+    "8th-order": {"order": 8, "tol": 1e-4},
+    "Chained BLA": {"order": 32, "tol": 1e-6},
+}
+
+GLITCH_CORRECTION_MODES = ["Off (Single-Ref)", "Pauldelbrot", "Strict Metric"]
+
+HISTOGRAM_SCHEME = 11
+HISTOGRAM_BINS = 65536
+PALETTE_LUT_SIZE = 2048
+BLA_LEVELS = 16
+
+
+def decimal_prec_for(plot_width_dec, extra_digits=150):
+    """Digits the Decimal grid arithmetic runs at: max(500, -adjusted(w) + 150)
+    (ensure_decimal_precision, config.py:63-74)."""
+    try:
+        p = Decimal(str(plot_width_dec))
+        if p > 0:
+            return max(DEFAULT_DECIMAL_PREC, -int(p.adjusted()) + extra_digits)
+    except (decimal.InvalidOperation, ValueError):
+        pass
+    return DEFAULT_DECIMAL_PREC
+
+
+def decimal_context(plot_width_dec):
+    ctx = decimal.Context(prec=decimal_prec_for(plot_width_dec))
+    return decimal.localcontext(ctx)
+
+
+def compute_required_prec_bits(plot_width_dec):
+    """Orbit precision in bits: -adjusted(w)*log2(10) + 128 guard bits, at
+    least 64 (config.py:36-48)."""
+    try:
+        p = Decimal(str(plot_width_dec))
+        if p > 0:
+            adj = -int(p.adjusted())
+            return max(64, int(adj * 3.321928094887362) + 128)
+        return 256
+    except (decimal.InvalidOperation, ValueError):
+        return 256
+
+
+def compute_floatexp_scale(plot_width_dec):
+    """E such that dc * 2^E is O(1): floor(-adjusted(w) * log2 10), >= 0
+    (config.py:51-60)."""
+    try:
+        p = Decimal(str(plot_width_dec))
+        if p <= 0:
+            return 0
+        return max(0, int(-p.adjusted() * 3.32192809488736235))
+    except (decimal.InvalidOperation, ValueError):
+        return 0
+
+
+@dataclass
+class RenderParameters:
+    """Every argument of one render call (config.py:196-243)."""
+
+    target_w: int = 1280
+    target_h: int = 720
+    max_iter: int = 1000
+    center_x_dec: Decimal = Decimal("-0.75")
+    center_y_dec: Decimal = Decimal("0.0")
+    plot_width_dec: Decimal = Decimal("4.0")
+    cmap_name: str = "inferno"
+    ssaa_factor: int = 1
+    edge_threshold: float = 0.35
+    color_scheme_id: int = 0
+    precision_mode: str = "1e-300 (Perturbation)"
+    fractal_type: int = 0
+    julia_cx: float = 0.0
+    julia_cy: float = 0.0
+    bbsa_tol: float = 1e-4
+    bbsa_order: int = 4
+    glitch_mode: str = "Off (Single-Ref)"
+    gen_n: int = 3
+    gen_kr: float = 0.25
+    gen_ki: float = 1.0
+    d_rgb_buf: object = None
+    h_rgb_buf: object = None
+    stream: object = 0
+    is_hdr: bool = False
+    compute_device: str = "GPU"
+    cpu_threads: int = 2
+    palette_offset: float = 0.0
+    color_density: float = 1.0
+    color_contrast: float = 1.0
+    sync_stream: bool = True
+    cancel_token: object = None
+    d_iter_buf: object = None
+    h_iter_buf: object = None
+    stream_slot: int = 0
+
+    def to_dict(self):
+        return asdict(self)
+
+    def as_dict(self):
+        return asdict(self)
+
+    def __getitem__(self, item):
+        return getattr(self, item)
+
+    def get(self, key, default=None):
+        return getattr(self, key, default)

@@ -1,0 +1,333 @@
+// The C ABI of libapeiron_hip.so (declared in include/apeiron_hip.h): a
+// context per device/stream, argument checks, and the entry points that
+// launch the kernels of ap_direct.hip, ap_pert.hip, ap_bla.hip and
+// ap_color.hip.  Nothing here throws; every failure is an int status with the
+// text kept in the context.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+
+#include <hip/hip_runtime.h>
+
+#include "apeiron_hip.h"
+#include "ap_launch.hpp"
+
+struct ap_ctx {
+    int device;
+    hipStream_t stream;
+    int lds_per_block;   // bytes of LDS one workgroup may use
+    int num_cus;
+    int clock_khz;
+    char name[256];
+    char err[512];
+    // Library-owned scratch, grown on demand and freed by ap_destroy.
+    void* compact_scratch;
+    size_t compact_bytes;
+    void* orbit32;
+    size_t orbit32_bytes;
+};
+
+namespace {
+
+int fail(ap_ctx* c, int code, const char* fmt, ...) {
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof c->err, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+int check(ap_ctx* c, hipError_t e, const char* what) {
+    if (e == hipSuccess) return AP_OK;
+    return fail(c, AP_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int use_device(ap_ctx* c) { return check(c, hipSetDevice(c->device), "hipSetDevice"); }
+
+int grow(ap_ctx* c, void** buf, size_t* have, size_t need, const char* what) {
+    if (*have >= need) return AP_OK;
+    if (*buf) {
+        hipError_t e = hipFree(*buf);
+        *buf = nullptr;
+        *have = 0;
+        if (e != hipSuccess) return check(c, e, what);
+    }
+    int r = check(c, hipMalloc(buf, need), what);
+    if (r == AP_OK) *have = need;
+    return r;
+}
+
+int check_view(ap_ctx* c, const ap_view* v, const ap_outputs* o) {
+    if (!v || !o) return fail(c, AP_ERR_ARG, "null view or outputs");
+    if (v->width <= 0 || v->height <= 0) return fail(c, AP_ERR_ARG, "bad size %dx%d", v->width, v->height);
+    if ((long long)v->width * v->height > (1LL << 31) - 1)
+        return fail(c, AP_ERR_ARG, "frame of %dx%d exceeds 2^31 pixels", v->width, v->height);
+    if (v->factor < 1 || v->factor > 64) return fail(c, AP_ERR_ARG, "bad ssaa factor %d", v->factor);
+    if (v->max_iter < 0) return fail(c, AP_ERR_ARG, "negative max_iter");
+    if (v->fractal_type < 0 || v->fractal_type > 4) return fail(c, AP_ERR_ARG, "bad fractal type %d", v->fractal_type);
+    if (v->fractal_type == AP_GENERAL_MANDELBROT && (v->gen_n < 1 || v->gen_n > 64))
+        return fail(c, AP_ERR_ARG, "bad gen_n %d", v->gen_n);
+    if (v->mode != AP_MODE_CENTRE && v->mode != AP_MODE_ALL_SUBSAMPLES && v->mode != AP_MODE_EDGE_SUBSAMPLES)
+        return fail(c, AP_ERR_ARG, "bad mode %d", v->mode);
+    if (v->mode == AP_MODE_EDGE_SUBSAMPLES && v->rebase_pass == 0 && !o->mask && !o->queue)
+        return fail(c, AP_ERR_ARG, "edge-subsample pass needs a mask or a queue");
+    if (v->rebase_pass > 0 && !o->mask && !o->queue)
+        return fail(c, AP_ERR_ARG, "rebase pass needs a glitch mask or a queue");
+    if (o->queue && !o->queue_count) return fail(c, AP_ERR_ARG, "queue without a device count");
+    if (o->rgb && !o->lut) return fail(c, AP_ERR_ARG, "rgb output needs a LUT");
+    if (o->cdf && o->num_bins < 2) return fail(c, AP_ERR_ARG, "cdf needs num_bins >= 2");
+    if (v->scheme_id < 0 || v->scheme_id > 11) return fail(c, AP_ERR_ARG, "bad scheme %d", v->scheme_id);
+    return AP_OK;
+}
+
+int check_orbit(ap_ctx* c, const ap_view* v, const ap_orbit* orb, const void* bla) {
+    if (!orb || !orb->re || !orb->im) return fail(c, AP_ERR_ARG, "null orbit");
+    if (orb->len < 0 || orb->len != v->ref_len)
+        return fail(c, AP_ERR_ARG, "orbit length %d does not match view ref_len %d", orb ? orb->len : -1, v->ref_len);
+    if (v->bla_levels > 1) {
+        if (!bla) return fail(c, AP_ERR_ARG, "bla_levels=%d but no table", v->bla_levels);
+        if (v->bla_levels > 31) return fail(c, AP_ERR_ARG, "bla_levels=%d > 31", v->bla_levels);
+    }
+    return AP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ap_init(int device, ap_ctx** out) {
+    if (!out) return AP_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return AP_ERR_HIP;
+    if (device < 0 || device >= n) return AP_ERR_ARG;
+    ap_ctx* c = new (std::nothrow) ap_ctx();
+    if (!c) return AP_ERR_ARG;
+    memset(c, 0, sizeof *c);
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete c;
+        return AP_ERR_HIP;
+    }
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) != hipSuccess) {
+        delete c;
+        return AP_ERR_HIP;
+    }
+    c->num_cus = p.multiProcessorCount;
+    c->clock_khz = p.clockRate;
+    snprintf(c->name, sizeof c->name, "%s", p.gcnArchName);
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+        lds = 65536;
+    c->lds_per_block = lds;
+    *out = c;
+    return AP_OK;
+}
+
+int ap_destroy(ap_ctx* c) {
+    if (!c) return AP_OK;
+    (void)hipSetDevice(c->device);
+    if (c->compact_scratch) (void)hipFree(c->compact_scratch);
+    if (c->orbit32) (void)hipFree(c->orbit32);
+    delete c;
+    return AP_OK;
+}
+
+const char* ap_last_error(const ap_ctx* c) { return c ? c->err : "null context"; }
+
+int ap_set_stream(ap_ctx* c, void* s) {
+    if (!c) return AP_ERR_ARG;
+    c->stream = (hipStream_t)s;
+    return AP_OK;
+}
+
+int ap_sync(ap_ctx* c) {
+    if (!c) return AP_ERR_ARG;
+    int r = use_device(c);
+    if (r) return r;
+    return check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+int ap_alloc(ap_ctx* c, size_t bytes, void** p) {
+    if (!c || !p) return AP_ERR_ARG;
+    int r = use_device(c);
+    if (r) return r;
+    return check(c, hipMalloc(p, bytes ? bytes : 1), "hipMalloc");
+}
+
+int ap_free(ap_ctx* c, void* p) {
+    if (!c) return AP_ERR_ARG;
+    if (!p) return AP_OK;
+    int r = use_device(c);
+    if (r) return r;
+    return check(c, hipFree(p), "hipFree");
+}
+
+int ap_memcpy_h2d(ap_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || (!dst && bytes) || (!src && bytes)) return AP_ERR_ARG;
+    int r = use_device(c);
+    if (r) return r;
+    return check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync H2D");
+}
+
+int ap_memcpy_d2h(ap_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || (!dst && bytes) || (!src && bytes)) return AP_ERR_ARG;
+    int r = use_device(c);
+    if (r) return r;
+    return check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync D2H");
+}
+
+int ap_memset(ap_ctx* c, void* dst, int value, size_t bytes) {
+    if (!c || (!dst && bytes)) return AP_ERR_ARG;
+    int r = use_device(c);
+    if (r) return r;
+    return check(c, hipMemsetAsync(dst, value, bytes, c->stream), "hipMemsetAsync");
+}
+
+int ap_device_info(ap_ctx* c, int* num_cus, int* clock_khz, char* name, size_t name_len) {
+    if (!c) return AP_ERR_ARG;
+    if (num_cus) *num_cus = c->num_cus;
+    if (clock_khz) *clock_khz = c->clock_khz;
+    if (name && name_len) snprintf(name, name_len, "%s", c->name);
+    return AP_OK;
+}
+
+int ap_render_fp64(ap_ctx* c, const ap_view* v, const ap_outputs* o) {
+    if (!c) return AP_ERR_ARG;
+    int r = check_view(c, v, o);
+    if (r) return r;
+    if ((r = use_device(c))) return r;
+    return check(c, ap::launch_fp64(*v, *o, c->stream), "fp64 pass");
+}
+
+int ap_render_dd(ap_ctx* c, const ap_view* v, const ap_outputs* o) {
+    if (!c) return AP_ERR_ARG;
+    int r = check_view(c, v, o);
+    if (r) return r;
+    if ((r = use_device(c))) return r;
+    return check(c, ap::launch_dd(*v, *o, c->stream), "dd pass");
+}
+
+int ap_render_pert64(ap_ctx* c, const ap_view* v, const ap_orbit* orb, const void* bla,
+                     const ap_outputs* o) {
+    if (!c) return AP_ERR_ARG;
+    int r = check_view(c, v, o);
+    if (r) return r;
+    if ((r = check_orbit(c, v, orb, bla))) return r;
+    if ((r = use_device(c))) return r;
+    return check(c, ap::launch_pert64(*v, *orb, bla, *o, c->stream), "pert64 pass");
+}
+
+static int render_fexp(ap_ctx* c, int mant, const ap_view* v, const ap_orbit* orb,
+                       const void* bla, const ap_outputs* o) {
+    if (!c) return AP_ERR_ARG;
+    int r = check_view(c, v, o);
+    if (r) return r;
+    if ((r = check_orbit(c, v, orb, bla))) return r;
+    if ((r = use_device(c))) return r;
+    if (mant == 32) {
+        size_t need = (size_t)(orb->len + 1) * 8;
+        if ((r = grow(c, &c->orbit32, &c->orbit32_bytes, need, "orbit32 scratch"))) return r;
+    }
+    return check(c, ap::launch_fexp(mant, *v, *orb, bla, *o, c->orbit32, c->stream),
+                 mant == 32 ? "fexp32 pass" : "fexp64 pass");
+}
+
+int ap_render_fexp64(ap_ctx* c, const ap_view* v, const ap_orbit* orb, const void* bla,
+                     const ap_outputs* o) {
+    return render_fexp(c, 64, v, orb, bla, o);
+}
+
+int ap_render_fexp32(ap_ctx* c, const ap_view* v, const ap_orbit* orb, const void* bla,
+                     const ap_outputs* o) {
+    return render_fexp(c, 32, v, orb, bla, o);
+}
+
+size_t ap_bla_bytes(int32_t ref_len, int32_t levels) {
+    if (ref_len <= 0 || levels <= 0) return 0;
+    return ap::bla_bytes(ref_len, levels);
+}
+
+int ap_bla_build(ap_ctx* c, const ap_orbit* orb, int32_t type, int32_t n, double kr, double ki,
+                 double tol, int32_t levels, void* table) {
+    if (!c) return AP_ERR_ARG;
+    if (!orb || !orb->re || !orb->im || !table) return fail(c, AP_ERR_ARG, "null orbit or table");
+    if (levels < 2 || levels > 31) return fail(c, AP_ERR_ARG, "levels=%d outside [2, 31]", levels);
+    if (type != AP_MANDELBROT && type != AP_JULIA_MANDELBROT && type != AP_GENERAL_MANDELBROT)
+        return fail(c, AP_ERR_UNSUPPORTED, "no BLA for fractal type %d", type);
+    if (orb->len < 4 || !(tol > 0.0)) return fail(c, AP_ERR_ARG, "ref_len=%d or tol=%g too small", orb->len, tol);
+    if (type == AP_GENERAL_MANDELBROT && (n < 2 || n > 64)) return fail(c, AP_ERR_ARG, "bad gen_n %d", n);
+    int r = use_device(c);
+    if (r) return r;
+    return check(c, ap::launch_bla_build(*orb, type, n, kr, ki, tol, levels, table, c->stream), "bla build");
+}
+
+int ap_bla_export(ap_ctx* c, const void* table, int32_t ref_len, int32_t levels, double* out) {
+    if (!c) return AP_ERR_ARG;
+    if (!table || !out || ref_len <= 0 || levels <= 0) return fail(c, AP_ERR_ARG, "bad export arguments");
+    int r = use_device(c);
+    if (r) return r;
+    return check(c, ap::launch_bla_export(table, ref_len, levels, out, c->stream), "bla export");
+}
+
+int ap_edge_mask(ap_ctx* c, const float* it, int32_t w, int32_t h, float thr, uint8_t* mask) {
+    if (!c) return AP_ERR_ARG;
+    if (!it || !mask || w <= 0 || h <= 0) return fail(c, AP_ERR_ARG, "bad edge-mask arguments");
+    int r = use_device(c);
+    if (r) return r;
+    return check(c, ap::launch_edge_mask(it, w, h, thr, mask, c->stream), "edge mask");
+}
+
+int ap_compact(ap_ctx* c, const uint8_t* mask, int32_t n, int32_t* queue, int32_t* count) {
+    if (!c) return AP_ERR_ARG;
+    if (!mask || !queue || !count || n < 0) return fail(c, AP_ERR_ARG, "bad compact arguments");
+    int r = use_device(c);
+    if (r) return r;
+    size_t need = (size_t)(ap::compact_scratch_ints(n) + 1) * sizeof(int);
+    if ((r = grow(c, &c->compact_scratch, &c->compact_bytes, need, "compact scratch"))) return r;
+    return check(c, ap::launch_compact(mask, n, queue, count, (int*)c->compact_scratch, c->stream), "compact");
+}
+
+int ap_glitch_bins(ap_ctx* c, const uint8_t* mask, int32_t w, int32_t h, int32_t* bins) {
+    if (!c) return AP_ERR_ARG;
+    if (!mask || !bins || w <= 0 || h <= 0) return fail(c, AP_ERR_ARG, "bad glitch-bin arguments");
+    int r = use_device(c);
+    if (r) return r;
+    return check(c, ap::launch_glitch_bins(mask, w, h, bins, c->stream), "glitch bins");
+}
+
+int ap_hist_build(ap_ctx* c, const float* it, int32_t n, int32_t max_iter, int32_t num_bins,
+                  int32_t* hist) {
+    if (!c) return AP_ERR_ARG;
+    if (!it || !hist || n < 0 || num_bins < 2) return fail(c, AP_ERR_ARG, "bad histogram arguments");
+    int r = use_device(c);
+    if (r) return r;
+    return check(c, ap::launch_hist_build(it, n, max_iter, num_bins, hist, c->lds_per_block, c->stream),
+                 "histogram");
+}
+
+int ap_hist_cdf(ap_ctx* c, const int32_t* hist, int32_t num_bins, float* cdf) {
+    if (!c) return AP_ERR_ARG;
+    if (!hist || !cdf || num_bins < 2) return fail(c, AP_ERR_ARG, "bad cdf arguments");
+    int r = use_device(c);
+    if (r) return r;
+    return check(c, ap::launch_hist_cdf(hist, num_bins, cdf, c->stream), "cdf");
+}
+
+int ap_colorize(ap_ctx* c, const ap_view* v, const float* it, const float* cdf, int32_t num_bins,
+                const float* lut, void* rgb) {
+    if (!c) return AP_ERR_ARG;
+    if (!v || !it || !lut || !rgb || v->width <= 0 || v->height <= 0)
+        return fail(c, AP_ERR_ARG, "bad colorize arguments");
+    if (cdf && num_bins < 2) return fail(c, AP_ERR_ARG, "cdf needs num_bins >= 2");
+    int r = use_device(c);
+    if (r) return r;
+    return check(c, ap::launch_colorize(*v, it, cdf, num_bins, lut, rgb, c->stream), "colorize");
+}
+
+}  // extern "C"

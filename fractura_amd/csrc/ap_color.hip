@@ -1,0 +1,330 @@
+// Colouring and mask kernels: edge detection, ordered mask compaction,
+// 16x16 glitch bins, the LDS-privatised 65536-bin histogram, its CDF scan and
+// the colouring pass.  These are HBM-bound byte/integer kernels; none of them
+// uses MFMA.
+#include "ap_device.hpp"
+#include "ap_launch.hpp"
+
+namespace ap {
+
+// ------------------------------------------------------------ edge mask --
+// 8-neighbour rule of edge_detect_kernel (math_core/cuda_kernels.py:
+// 1587-1611): an edge where a neighbour differs in interior/exterior status
+// or, for an exterior pixel, in smooth value by more than the threshold.
+__global__ __launch_bounds__(256) void edge_mask_kernel(const float* __restrict__ it, int w,
+                                                        int h, float thr,
+                                                        uint8_t* __restrict__ mask) {
+    int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= w || y >= h) return;
+    float s = it[(long long)y * w + x];
+    bool edge = false;
+    for (int dy = -1; dy <= 1 && !edge; ++dy) {
+        int ny = y + dy;
+        if (ny < 0 || ny >= h) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+            int nx = x + dx;
+            if (nx < 0 || nx >= w || (dx == 0 && dy == 0)) continue;
+            float sn = it[(long long)ny * w + nx];
+            if ((s < 0.0f) != (sn < 0.0f) || (s >= 0.0f && fabsf(s - sn) > thr)) {
+                edge = true;
+                break;
+            }
+        }
+    }
+    mask[(long long)y * w + x] = edge ? 1 : 0;
+}
+
+hipError_t launch_edge_mask(const float* it, int w, int h, float thr, uint8_t* mask,
+                            hipStream_t s) {
+    dim3 grid((w + 63) / 64, (h + 3) / 4);
+    hipLaunchKernelGGL(edge_mask_kernel, grid, dim3(256), 0, s, it, w, h, thr, mask);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ compaction --
+// Ordered stream compaction of a byte mask in three launches: per-block
+// counts (wave ballots), one-block exclusive scan of the counts, then each
+// block writes its indices at its offset plus the in-block ballot prefix.
+// The result equals np.flatnonzero(mask) (math_core/engine.py:1905).
+constexpr int kCompactItems = 4096;  // mask bytes per block (16 per lane)
+
+__device__ inline int wave_prefix(unsigned long long ballot) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(ballot >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((unsigned)ballot, 0u));
+}
+
+__global__ __launch_bounds__(256) void compact_count_kernel(const uint8_t* __restrict__ mask,
+                                                            int n, int* __restrict__ counts) {
+    __shared__ int wave_sum[4];
+    int base = blockIdx.x * kCompactItems;
+    int total = 0;
+    for (int r = 0; r < kCompactItems / 256; ++r) {
+        int i = base + r * 256 + threadIdx.x;
+        bool on = i < n && mask[i] != 0;
+        unsigned long long b = __ballot(on);
+        total += __popcll(b);  // wave-uniform
+    }
+    if ((threadIdx.x & 63) == 0) wave_sum[threadIdx.x >> 6] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = wave_sum[0] + wave_sum[1] + wave_sum[2] + wave_sum[3];
+}
+
+// Exclusive scan of `nb` block counts by one block of 1024 lanes; the total
+// goes to *count.
+__global__ __launch_bounds__(1024) void compact_scan_kernel(int* __restrict__ counts, int nb,
+                                                            int* __restrict__ count) {
+    __shared__ int part[1024];
+    int per = (nb + 1023) / 1024;
+    int lo = threadIdx.x * per;
+    int sum = 0;
+    for (int i = 0; i < per; ++i)
+        if (lo + i < nb) sum += counts[lo + i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = part[threadIdx.x] - sum;  // exclusive prefix of this lane's range
+    for (int i = 0; i < per; ++i) {
+        if (lo + i < nb) {
+            int c = counts[lo + i];
+            counts[lo + i] = run;
+            run += c;
+        }
+    }
+    if (threadIdx.x == 1023) *count = part[1023];
+}
+
+__global__ __launch_bounds__(256) void compact_write_kernel(const uint8_t* __restrict__ mask,
+                                                            int n,
+                                                            const int* __restrict__ offsets,
+                                                            int* __restrict__ queue) {
+    __shared__ int wave_cnt[4];
+    int base = blockIdx.x * kCompactItems;
+    int out = offsets[blockIdx.x];
+    int wave = threadIdx.x >> 6;
+    for (int r = 0; r < kCompactItems / 256; ++r) {
+        int i = base + r * 256 + threadIdx.x;
+        bool on = i < n && mask[i] != 0;
+        unsigned long long b = __ballot(on);
+        if ((threadIdx.x & 63) == 0) wave_cnt[wave] = __popcll(b);
+        __syncthreads();
+        int before = 0;
+        for (int w2 = 0; w2 < wave; ++w2) before += wave_cnt[w2];
+        int round_total = wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
+        if (on) queue[out + before + wave_prefix(b)] = i;
+        out += round_total;
+        __syncthreads();
+    }
+}
+
+int compact_scratch_ints(int n) { return (n + kCompactItems - 1) / kCompactItems; }
+
+hipError_t launch_compact(const uint8_t* mask, int n, int* queue, int* count, int* scratch,
+                          hipStream_t s) {
+    int nb = compact_scratch_ints(n);
+    if (nb == 0) return hipMemsetAsync(count, 0, sizeof(int), s);
+    hipLaunchKernelGGL(compact_count_kernel, dim3(nb), dim3(256), 0, s, mask, n, scratch);
+    hipLaunchKernelGGL(compact_scan_kernel, dim3(1), dim3(1024), 0, s, scratch, nb, count);
+    hipLaunchKernelGGL(compact_write_kernel, dim3(nb), dim3(256), 0, s, mask, n, scratch, queue);
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------------- glitch bins --
+// Counts of mask entries per 16x16 spatial bin, bin = clip(row*16//H) * 16 +
+// clip(col*16//W) (math_core/engine.py:1924-1928).
+__global__ __launch_bounds__(256) void glitch_bins_kernel(const uint8_t* __restrict__ mask,
+                                                          int w, int h, int* __restrict__ bins) {
+    __shared__ int lb[256];
+    lb[threadIdx.x] = 0;
+    __syncthreads();
+    long long n = (long long)w * h;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (long long)gridDim.x * 256) {
+        if (mask[i]) {
+            int row = (int)(i / w), col = (int)(i % w);
+            int br = min(max((row * 16) / h, 0), 15);
+            int bc = min(max((col * 16) / w, 0), 15);
+            atomicAdd(&lb[br * 16 + bc], 1);
+        }
+    }
+    __syncthreads();
+    if (lb[threadIdx.x]) atomicAdd(&bins[threadIdx.x], lb[threadIdx.x]);
+}
+
+hipError_t launch_glitch_bins(const uint8_t* mask, int w, int h, int* bins, hipStream_t s) {
+    hipError_t e = hipMemsetAsync(bins, 0, 256 * sizeof(int), s);
+    if (e != hipSuccess) return e;
+    long long n = (long long)w * h;
+    long long nb = (n + 255) / 256;
+    int blocks = (int)(nb < 1024 ? nb : 1024);
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(glitch_bins_kernel, dim3(blocks), dim3(256), 0, s, mask, w, h, bins);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------- histogram --
+// bin = int(s / max(1, max_iter) * (bins-1)), clamped, escaped samples only
+// (build_histogram_kernel, math_core/cuda_kernels.py:1999-2013).
+__device__ inline int hist_bin(float s, double inv_max, int num_bins) {
+    double pos = ((double)s * inv_max) * (double)(num_bins - 1);
+    int b = (int)pos;
+    return b >= num_bins ? num_bins - 1 : (b < 0 ? 0 : b);
+}
+
+// LDS-privatised build: each block owns a private copy of all bins as 16-bit
+// counters packed two per 32-bit LDS word (65536 bins = 128 KiB of LDS), counts
+// a contiguous slice of at most 65535 pixels with LDS atomics, then flushes
+// only its non-zero bins to the global histogram.  Global atomics per block are
+// bounded by the number of distinct bins it saw, not by its pixel count.
+constexpr int kHistSlice = 32768;  // pixels per block (< 65536: no 16-bit carry)
+
+__global__ __launch_bounds__(1024) void hist_lds_kernel(const float* __restrict__ it, int n,
+                                                        double inv_max, int num_bins,
+                                                        int* __restrict__ hist) {
+    extern __shared__ unsigned int packed[];
+    const int words = (num_bins + 1) >> 1;
+    for (int i = threadIdx.x; i < words; i += 1024) packed[i] = 0u;
+    __syncthreads();
+    long long base = (long long)blockIdx.x * kHistSlice;
+    for (int r = threadIdx.x; r < kHistSlice; r += 1024) {
+        long long i = base + r;
+        if (i < n) {
+            float s = it[i];
+            if (s >= 0.0f) {
+                int b = hist_bin(s, inv_max, num_bins);
+                atomicAdd(&packed[b >> 1], 1u << ((b & 1) * 16));
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < words; i += 1024) {
+        unsigned int wv = packed[i];
+        if (wv) {
+            unsigned int lo = wv & 0xffffu, hi = wv >> 16;
+            if (lo) atomicAdd(&hist[2 * i], (int)lo);
+            if (hi && 2 * i + 1 < num_bins) atomicAdd(&hist[2 * i + 1], (int)hi);
+        }
+    }
+}
+
+// Fallback when the device cannot give a block the LDS: global atomics.
+__global__ __launch_bounds__(256) void hist_global_kernel(const float* __restrict__ it, int n,
+                                                          double inv_max, int num_bins,
+                                                          int* __restrict__ hist) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (long long)gridDim.x * 256) {
+        float s = it[i];
+        if (s >= 0.0f) atomicAdd(&hist[hist_bin(s, inv_max, num_bins)], 1);
+    }
+}
+
+hipError_t launch_hist_build(const float* it, int n, int max_iter, int num_bins, int* hist,
+                             int lds_bytes_available, hipStream_t s) {
+    hipError_t e = hipMemsetAsync(hist, 0, (size_t)num_bins * sizeof(int), s);
+    if (e != hipSuccess) return e;
+    if (n <= 0) return hipSuccess;
+    double inv_max = 1.0 / (max_iter > 1 ? (double)max_iter : 1.0);
+    size_t lds = (size_t)((num_bins + 1) / 2) * sizeof(unsigned int);
+    if ((long long)lds <= (long long)lds_bytes_available) {
+        static bool attr_set = false;
+        if (!attr_set && lds > 65536) {
+            e = hipFuncSetAttribute((const void*)hist_lds_kernel,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+            attr_set = true;
+        }
+        int blocks = (n + kHistSlice - 1) / kHistSlice;
+        hipLaunchKernelGGL(hist_lds_kernel, dim3(blocks), dim3(1024), lds, s, it, n, inv_max,
+                           num_bins, hist);
+    } else {
+        int blocks = min(4096, (n + 255) / 256);
+        hipLaunchKernelGGL(hist_global_kernel, dim3(blocks), dim3(256), 0, s, it, n, inv_max,
+                           num_bins, hist);
+    }
+    return hipGetLastError();
+}
+
+// CDF: inclusive scan of the bins, divided by the total and rounded to
+// float32; a linear ramp when nothing escaped (finalize_normalized_cdf_cuda_
+// kernel, math_core/cuda_kernels.py:1977-1996).  Integer sums are exact, as
+// the reference's float64 sums of integer counts below 2^53 are.
+__global__ __launch_bounds__(1024) void hist_cdf_kernel(const int* __restrict__ hist,
+                                                        int num_bins, float* __restrict__ cdf) {
+    __shared__ long long part[1024];
+    int per = (num_bins + 1023) / 1024;
+    int lo = threadIdx.x * per;
+    long long sum = 0;
+    for (int i = 0; i < per; ++i)
+        if (lo + i < num_bins) sum += hist[lo + i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        long long v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    long long total = part[1023];
+    long long run = part[threadIdx.x] - sum;
+    for (int i = 0; i < per; ++i) {
+        int b = lo + i;
+        if (b < num_bins) {
+            run += hist[b];
+            cdf[b] = total > 0 ? (float)((double)run / (double)total)
+                               : (float)((double)b / (double)(num_bins - 1));
+        }
+    }
+}
+
+hipError_t launch_hist_cdf(const int* hist, int num_bins, float* cdf, hipStream_t s) {
+    hipLaunchKernelGGL(hist_cdf_kernel, dim3(1), dim3(1024), 0, s, hist, num_bins, cdf);
+    return hipGetLastError();
+}
+
+// -------------------------------------------------------------- colourise --
+// RGB from a cached float32 map: histogram CDF (scheme 11, apply_histogram_
+// equalization_kernel, math_core/cuda_kernels.py:2016-2035) or a palette
+// transfer function (recolor_iter_map_cuda_kernel, :3201-3222).  Four pixels
+// per lane: one 16-byte load of the map and 12 (or 24) contiguous RGB bytes.
+__global__ __launch_bounds__(256) void colorize_kernel(ap_view v, const float* __restrict__ it,
+                                                       const float* __restrict__ cdf,
+                                                       int num_bins,
+                                                       const float* __restrict__ lut,
+                                                       void* __restrict__ rgb, long long n) {
+    long long p0 = ((long long)blockIdx.x * 256 + threadIdx.x) * 4;
+    if (p0 >= n) return;
+    float s4[4];
+    if (p0 + 4 <= n) {
+        float4 q = *(const float4*)(it + p0);
+        s4[0] = q.x; s4[1] = q.y; s4[2] = q.z; s4[3] = q.w;
+    } else {
+        for (int t = 0; t < 4; ++t) s4[t] = p0 + t < n ? it[p0 + t] : -1.0f;
+    }
+    ap_outputs o = {};
+    o.cdf = cdf;
+    o.num_bins = num_bins;
+    o.lut = lut;
+    for (int t = 0; t < 4 && p0 + t < n; ++t) {
+        double r = 0.0, g = 0.0, b = 0.0;
+        if (s4[t] >= 0.0f) sample_colour(v, o, s4[t], r, g, b);
+        store_rgb(rgb, v.is_hdr, p0 + t, r, g, b);
+    }
+}
+
+hipError_t launch_colorize(const ap_view& v, const float* it, const float* cdf, int num_bins,
+                           const float* lut, void* rgb, hipStream_t s) {
+    long long n = (long long)v.width * v.height;
+    if (n <= 0) return hipSuccess;
+    long long threads = (n + 3) / 4;
+    int blocks = (int)((threads + 255) / 256);
+    hipLaunchKernelGGL(colorize_kernel, dim3(blocks), dim3(256), 0, s, v, it, cdf, num_bins, lut,
+                       rgb, n);
+    return hipGetLastError();
+}
+
+}  // namespace ap

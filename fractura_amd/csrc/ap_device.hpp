@@ -1,0 +1,309 @@
+// Device-side building blocks shared by every pixel engine: double-double
+// arithmetic, the escape/smooth epilogue, palette transfer functions, the
+// LUT lerp and the tile/queue mapping of work items to pixels.
+//
+// All code here is compiled with -ffp-contract=off: a multiply followed by an
+// add stays two correctly rounded operations unless fma() is written out, so
+// the FP64 engines reproduce the reference CPU arithmetic operation by
+// operation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "apeiron_hip.h"
+
+namespace ap {
+
+constexpr double kBailout2 = 65536.0;  // |z|^2 > 65536, R = 256 (math_core/cpu_kernels.py:246)
+constexpr int kLutSize = 2048;
+
+// ---------------------------------------------------------------- results --
+struct Sample {
+    int count;      // escape index, max_iter if never escaped, -1 for shortcut
+    float s;        // reference float32 smooth value (-1 for the interior)
+    double smooth;  // float64 smooth value (-1 for the interior)
+    int glitch;     // perturbation glitch flag
+    long long iters;  // iterations advanced (BLA leaps count their length)
+    int plain;      // executed single-iteration steps
+    int bla;        // executed BLA leaps
+};
+
+__device__ inline Sample interior(int count, long long iters) {
+    Sample r;
+    r.count = count;
+    r.s = -1.0f;
+    r.smooth = -1.0;
+    r.glitch = 0;
+    r.iters = iters;
+    r.plain = 0;
+    r.bla = 0;
+    return r;
+}
+
+// Escape epilogue of every reference sample evaluator
+// (math_core/cpu_kernels.py:246-249): the float32 value is
+// f32(k+1) - f32((ln(0.5 ln|z|^2) - LN_BAILOUT_LOG) * inv_ln) with float32
+// subtraction; the float64 value keeps the same chain in double and uses the
+// exact 1/ln n for type 4 (the reference rounds it to float32).
+__device__ inline Sample escaped(int k, double mag2, double inv_ln_ref,
+                                 double inv_ln_exact, double ln_bailout_log) {
+    Sample r;
+    r.count = k;
+    r.glitch = 0;
+    r.iters = k;
+    r.plain = 0;
+    r.bla = 0;
+    double log_mag = log(mag2);
+    if (log_mag > 1.0) {
+        double nu = log(0.5 * log_mag) - ln_bailout_log;
+        r.s = (float)(k + 1.0) - (float)(nu * inv_ln_ref);
+        r.smooth = (double)(k + 1) - nu * inv_ln_exact;
+    } else {
+        r.s = (float)k;
+        r.smooth = (double)k;
+    }
+    return r;
+}
+
+// Type 0 cardioid / period-2 bulb shortcut (math_core/cpu_kernels.py:229-234).
+__device__ inline bool in_main_components(double x, double y) {
+    double xm = x - 0.25;
+    double y2 = y * y;
+    double q = xm * xm + y2;
+    return (q * (q + xm) <= 0.25 * y2) || ((x + 1.0) * (x + 1.0) + y2 <= 0.0625);
+}
+
+// ---------------------------------------------------------- double-double --
+// Knuth two-sum and the exact product through fma; the reference CPU twin
+// splits with Dekker's 134217729 constant (math_core/cpu_kernels.py:103-113),
+// which yields the same exact error term.
+struct dd {
+    double hi, lo;
+};
+
+__device__ inline dd two_sum(double a, double b) {
+    double s = a + b;
+    double v = s - a;
+    double e = (a - (s - v)) + (b - v);
+    return {s, e};
+}
+__device__ inline dd two_prod(double a, double b) {
+    double p = a * b;
+    return {p, fma(a, b, -p)};
+}
+__device__ inline dd dd_add(dd a, dd b) {
+    dd t = two_sum(a.hi, b.hi);
+    return two_sum(t.hi, (t.lo + a.lo) + b.lo);
+}
+__device__ inline dd dd_sub(dd a, dd b) {
+    dd t = two_sum(a.hi, -b.hi);
+    return two_sum(t.hi, (t.lo + a.lo) - b.lo);
+}
+__device__ inline dd dd_mul(dd a, dd b) {
+    dd p = two_prod(a.hi, b.hi);
+    return two_sum(p.hi, (p.lo + a.hi * b.lo) + a.lo * b.hi);
+}
+__device__ inline dd dd_mul_f64(dd a, double b) {
+    dd p = two_prod(a.hi, b);
+    return two_sum(p.hi, p.lo + a.lo * b);
+}
+__device__ inline dd dd_sqr(dd a) {
+    dd p = two_prod(a.hi, a.hi);
+    return two_sum(p.hi, p.lo + (2.0 * a.hi) * a.lo);
+}
+__device__ inline dd dd_abs(dd a) {
+    return (a.hi < 0.0 || (a.hi == 0.0 && a.lo < 0.0)) ? dd{-a.hi, -a.lo} : a;
+}
+
+// ---------------------------------------------------------------- colour --
+// Python's float modulo (result takes the divisor's sign).
+__device__ inline double pymod(double a, double b) {
+    double m = fmod(a, b);
+    if (m != 0.0 && ((b < 0.0) != (m < 0.0))) m += b;
+    return m;
+}
+
+__device__ inline double tri_wave(double phase) {
+    return 1.0 - fabs((phase < 0.0 ? phase + 2.0 : phase) - 1.0);
+}
+
+// Palette transfer functions 0..10 (and 11 and above fall back to 0), as in
+// cpu_palette_norm (math_core/cpu_kernels.py:14-78).  The square roots of the
+// float32 sample are float32 operations there (Numba types math.sqrt of a
+// float32 as float32); everything else is double.
+__device__ inline double palette_norm(float s, int scheme, double off, double density,
+                                      double contrast) {
+    double dens = density <= 0.0 ? 1.0 : density;
+    double cont = contrast <= 0.0 ? 1.0 : contrast;
+    double sd = (double)s;
+    switch (scheme) {
+    case 1: {
+        double ph = pymod(log(sd * cont + 1.0) * (0.45 * dens) + off * 2.0, 2.0);
+        return tri_wave(ph);
+    }
+    case 2: {
+        double ph = pymod((double)sqrtf(s) * (0.125 * dens) + off * 2.0, 2.0);
+        double t = tri_wave(ph);
+        double k = 4.0 * cont;
+        double td = tanh(0.5 * k);
+        double scale = fabs(td) > 1e-7 ? 0.5 / td : 0.5186574;
+        return 0.5 + tanh((t - 0.5) * k) * scale;
+    }
+    case 3: {
+        double ph = pymod((double)sqrtf(s) * (0.125 * dens) + off * 2.0, 2.0);
+        double t = tri_wave(ph);
+        double k = 3.0 * cont;
+        double ed = exp(k) - 1.0;
+        double scale = fabs(ed) > 1e-7 ? 1.0 / ed : 0.0523957;
+        return (exp(k * t) - 1.0) * scale;
+    }
+    case 4: {
+        double ph = pymod((double)sqrtf(s) * (0.5 * dens) + off * 2.0, 2.0);
+        double t = tri_wave(ph);
+        return cont != 1.0 ? pow(t, cont) : t;
+    }
+    case 5: {
+        double ang = sd * (0.05 * dens) + off * 6.28318530718;
+        double ratio = 1.61803398875 * cont;
+        return 0.5 + 0.25 * (sin(ang) + sin(ang * ratio));
+    }
+    case 6: {
+        double ang = sd * (0.08 * dens) + off * 6.28318530718;
+        double v = 0.5 * (1.0 - cos(ang));
+        return cont != 1.0 ? pow(v, cont) : v;
+    }
+    case 7: {
+        double ph = sd * (0.05 * dens) + off * 6.28318530718;
+        double b = 0.5 * cont;
+        double wave = sin(ph) + b * sin(ph * 2.0 + 1.2) + (b * b) * sin(ph * 4.0 + 2.4) +
+                      (b * b * b) * sin(ph * 8.0 + 3.6);
+        double ns = 1.0 + b + b * b + b * b * b;
+        double scale = ns > 1e-7 ? 0.5 / ns : 0.26666666666666666;
+        return 0.5 + scale * wave;
+    }
+    case 8: {
+        double ph = pymod((double)sqrtf(s) * (0.1 * dens) + off * 2.0, 2.0);
+        double t = tri_wave(ph);
+        if (cont != 1.0) t = pow(t, cont);
+        double t2 = t * t;
+        return t2 * t * (t * (t * 6.0 - 15.0) + 10.0);
+    }
+    case 9: {
+        double ang = sd * (0.04 * dens) + (sd * sd) * (0.00008 * dens * cont) +
+                     off * 6.28318530718;
+        return 0.5 + 0.5 * sin(ang);
+    }
+    case 10: {
+        double raw = pymod(1.0 - exp(-0.06 * cont * sqrt(sd * dens)) + off, 1.0);
+        return raw >= 0.0 ? raw : raw + 1.0;
+    }
+    default: {  // 0 and > 10
+        double ph = pymod((double)sqrtf(s) * (0.125 * dens) + off * 2.0, 2.0);
+        double t = tri_wave(ph);
+        return cont != 1.0 ? pow(t, cont) : t;
+    }
+    }
+}
+
+// Histogram-equalised transfer (sample_cdf_norm, math_core/cuda_kernels.py:
+// 236-259): CDF lerp at s/max_iter*(bins-1), the difference of the two CDF
+// entries taken in float32 as the reference does.
+__device__ inline double cdf_norm(float s, int max_iter, const float* cdf, int num_bins,
+                                  double off, double density, double contrast) {
+    double inv_max = 1.0 / fmax(1.0, (double)max_iter);
+    double pos = ((double)s * inv_max) * (double)(num_bins - 1);
+    int k = (int)pos;
+    double c;
+    if (k >= num_bins - 1) {
+        c = cdf[num_bins - 1];
+    } else if (k < 0) {
+        c = cdf[0];
+    } else {
+        double frac = pos - (double)k;
+        float c0 = cdf[k];
+        float c1 = cdf[k + 1];
+        c = (double)c0 + frac * (double)(c1 - c0);
+    }
+    if (contrast != 1.0 && contrast > 0.0) c = pow(c, contrast);
+    double dens = density <= 0.0 ? 1.0 : density;
+    double ph = pymod(c * (2.0 * dens) + off * 2.0, 2.0);
+    return tri_wave(ph);
+}
+
+// 2048-entry LUT lerp (cpu_sample_lut_lerp, math_core/cpu_kernels.py:81-92).
+__device__ inline void lut_lerp(const float* __restrict__ lut, double norm, double& r,
+                                double& g, double& b) {
+    double n = norm < 0.0 ? 0.0 : (norm > 1.0 ? 1.0 : norm);
+    double pos = n * 2047.0;
+    int i0 = (int)pos;
+    int i1 = i0 >= 2047 ? 2047 : i0 + 1;
+    double frac = pos - (double)i0;
+    double inv = 1.0 - frac;
+    r = (double)lut[i0 * 3 + 0] * inv + (double)lut[i1 * 3 + 0] * frac;
+    g = (double)lut[i0 * 3 + 1] * inv + (double)lut[i1 * 3 + 1] * frac;
+    b = (double)lut[i0 * 3 + 2] * inv + (double)lut[i1 * 3 + 2] * frac;
+}
+
+__device__ inline void store_rgb(void* rgb, int is_hdr, long long idx, double r, double g,
+                                 double b) {
+    // int(x + 0.5) truncates toward zero, as Python int() does.
+    if (is_hdr) {
+        uint16_t* p = (uint16_t*)rgb + idx * 3;
+        p[0] = (uint16_t)(int)(r + 0.5);
+        p[1] = (uint16_t)(int)(g + 0.5);
+        p[2] = (uint16_t)(int)(b + 0.5);
+    } else {
+        uint8_t* p = (uint8_t*)rgb + idx * 3;
+        p[0] = (uint8_t)(int)(r + 0.5);
+        p[1] = (uint8_t)(int)(g + 0.5);
+        p[2] = (uint8_t)(int)(b + 0.5);
+    }
+}
+
+// Colour of one sample for the RGB accumulators: palette scheme, or the
+// histogram CDF when one is given.
+__device__ inline void sample_colour(const ap_view& v, const ap_outputs& o, float s, double& r,
+                                     double& g, double& b) {
+    double norm = o.cdf ? cdf_norm(s, v.max_iter, o.cdf, o.num_bins, v.palette_offset,
+                                   v.color_density, v.color_contrast)
+                        : palette_norm(s, v.scheme_id, v.palette_offset, v.color_density,
+                                       v.color_contrast);
+    lut_lerp(o.lut, norm, r, g, b);
+}
+
+// ------------------------------------------------------- work-item mapping --
+// Blocks of 256 lanes cover 16x16 pixel tiles; each wave64 covers an 8x8
+// sub-tile, so the lanes of one wave sample neighbouring points whose escape
+// counts are similar (less divergence than a 64x1 strip).
+constexpr int kTile = 16;
+constexpr int kBlock = 256;
+
+__device__ inline bool tile_pixel(int width, int height, int& x, int& y) {
+    int lane = threadIdx.x & 63;
+    int wave = threadIdx.x >> 6;
+    x = blockIdx.x * kTile + (wave & 1) * 8 + (lane & 7);
+    y = blockIdx.y * kTile + (wave >> 1) * 8 + (lane >> 3);
+    return x < width && y < height;
+}
+
+// Wave-level reduction of the per-lane statistics, one atomic per wave.
+__device__ inline void flush_stats(unsigned long long* stats, unsigned long long iters,
+                                   unsigned long long plain, unsigned long long bla,
+                                   unsigned long long samples) {
+    if (!stats) return;
+    for (int off = 32; off > 0; off >>= 1) {
+        iters += __shfl_down(iters, off, 64);
+        plain += __shfl_down(plain, off, 64);
+        bla += __shfl_down(bla, off, 64);
+        samples += __shfl_down(samples, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (iters) atomicAdd(&stats[AP_STAT_ITERATIONS], iters);
+        if (plain) atomicAdd(&stats[AP_STAT_PLAIN_STEPS], plain);
+        if (bla) atomicAdd(&stats[AP_STAT_BLA_STEPS], bla);
+        if (samples) atomicAdd(&stats[AP_STAT_SAMPLES], samples);
+    }
+}
+
+}  // namespace ap

@@ -1,0 +1,40 @@
+// Host-side launchers implemented in the kernel translation units and called
+// by the C ABI in ap_api.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "apeiron_hip.h"
+
+namespace ap {
+
+hipError_t launch_fp64(const ap_view& v, const ap_outputs& o, hipStream_t s);
+hipError_t launch_dd(const ap_view& v, const ap_outputs& o, hipStream_t s);
+
+// Perturbation engines.  `mant` selects the FloatExp mantissa width (64/32).
+hipError_t launch_pert64(const ap_view& v, const ap_orbit& orb, const void* bla,
+                         const ap_outputs& o, hipStream_t s);
+hipError_t launch_fexp(int mant, const ap_view& v, const ap_orbit& orb, const void* bla,
+                       const ap_outputs& o, void* scratch_f32_orbit, hipStream_t s);
+
+// BLA table.
+size_t bla_bytes(int ref_len, int levels);
+hipError_t launch_bla_build(const ap_orbit& orb, int type, int n, double kr, double ki,
+                            double tol, int levels, void* table, hipStream_t s);
+hipError_t launch_bla_export(const void* table, int ref_len, int levels, double* out,
+                             hipStream_t s);
+
+// Colouring, edges, queues.
+hipError_t launch_edge_mask(const float* iter_map, int w, int h, float thr, uint8_t* mask,
+                            hipStream_t s);
+hipError_t launch_compact(const uint8_t* mask, int n, int* queue, int* count,
+                          int* block_scratch, hipStream_t s);
+int compact_scratch_ints(int n);
+hipError_t launch_glitch_bins(const uint8_t* mask, int w, int h, int* bins, hipStream_t s);
+hipError_t launch_hist_build(const float* iter_map, int n, int max_iter, int num_bins,
+                             int* hist, int lds_bytes_available, hipStream_t s);
+hipError_t launch_hist_cdf(const int* hist, int num_bins, float* cdf, hipStream_t s);
+hipError_t launch_colorize(const ap_view& v, const float* iter_map, const float* cdf,
+                           int num_bins, const float* lut, void* rgb, hipStream_t s);
+
+}  // namespace ap

@@ -1,0 +1,124 @@
+// Generic pixel pass: maps work items (tiles or a compacted queue) to pixels,
+// runs an engine's sample evaluator for the centre sample or the factor^2
+// subsamples, and writes counts / smooth values / the reference float32 map /
+// glitch flags / RGB.  The reference drivers are mandelbrot_unified_*_kernel
+// (math_core/cuda_kernels.py:1615-1917) and ssaa_resolve_hist_*_kernel
+// (:2038-2234).
+//
+// An engine is a struct with
+//   struct Data;                               // engine constants (orbit ...)
+//   static constexpr bool kIterInSubsamples;   // mode 1/2 rewrites iter_map
+//   __device__ static Sample sample(const ap_view&, const Data&, double si, double sj);
+#pragma once
+
+#include "ap_device.hpp"
+
+namespace ap {
+
+struct LaneStats {
+    unsigned long long iters = 0, plain = 0, bla = 0, samples = 0;
+    __device__ void add(const Sample& r) {
+        iters += (unsigned long long)(r.iters > 0 ? r.iters : 0);
+        plain += (unsigned long long)r.plain;
+        bla += (unsigned long long)r.bla;
+        samples += 1;
+    }
+};
+
+template <class Eng>
+__device__ inline void process_pixel(const ap_view& v, const ap_outputs& o,
+                                     const typename Eng::Data& d, int x, int y,
+                                     bool from_queue, LaneStats& st) {
+    const long long idx = (long long)y * v.width + x;
+    if (!from_queue) {
+        if (v.rebase_pass > 0 && o.mask && o.mask[idx] == 0) return;
+        if (v.mode == AP_MODE_EDGE_SUBSAMPLES && v.rebase_pass == 0 && o.mask[idx] == 0) return;
+    }
+    const int f = v.factor;
+    if (v.mode == AP_MODE_CENTRE) {
+        const double off = ((double)f - 1.0) * 0.5;
+        Sample r = Eng::sample(v, d, (double)(x * f) + off, (double)(y * f) + off);
+        st.add(r);
+        if (o.counts) o.counts[idx] = r.count;
+        if (o.smooth) o.smooth[idx] = r.smooth;
+        if (o.iter_map) o.iter_map[idx] = r.s;
+        if (o.glitch) o.glitch[idx] = r.glitch ? 1 : 0;
+        if (o.rgb) {
+            double cr = 0.0, cg = 0.0, cb = 0.0;
+            if (r.s >= 0.0f) sample_colour(v, o, r.s, cr, cg, cb);
+            store_rgb(o.rgb, v.is_hdr, idx, cr, cg, cb);
+        }
+        return;
+    }
+    double ar = 0.0, ag = 0.0, ab = 0.0;
+    int glitches = 0;
+    Sample r;
+    for (int sy = 0; sy < f; ++sy) {
+        for (int sx = 0; sx < f; ++sx) {
+            r = Eng::sample(v, d, (double)(x * f + sx), (double)(y * f + sy));
+            st.add(r);
+            if (r.glitch) ++glitches;
+            if (r.s >= 0.0f) {
+                double cr, cg, cb;
+                sample_colour(v, o, r.s, cr, cg, cb);
+                ar += cr;
+                ag += cg;
+                ab += cb;
+            }
+        }
+    }
+    const double inv = 1.0 / (double)(f * f);
+    if (o.rgb) store_rgb(o.rgb, v.is_hdr, idx, ar * inv, ag * inv, ab * inv);
+    if (o.glitch) o.glitch[idx] = glitches > 0 ? 1 : 0;
+    // The perturbation drivers leave the last subsample in the float32 map
+    // (math_core/cuda_kernels.py:1824,1917); the direct drivers and the
+    // histogram resolve (cdf given) do not, except in rebase passes (:2173).
+    if ((Eng::kIterInSubsamples && !o.cdf) || v.rebase_pass > 0) {
+        if (o.iter_map) o.iter_map[idx] = r.s;
+        if (o.counts) o.counts[idx] = r.count;
+        if (o.smooth) o.smooth[idx] = r.smooth;
+    }
+}
+
+template <class Eng>
+__global__ __launch_bounds__(kBlock) void tile_pass(ap_view v, ap_outputs o,
+                                                   typename Eng::Data d) {
+    LaneStats st;
+    int x, y;
+    if (tile_pixel(v.width, v.height, x, y)) process_pixel<Eng>(v, o, d, x, y, false, st);
+    flush_stats(o.stats, st.iters, st.plain, st.bla, st.samples);
+}
+
+// Compacted queue: a grid-stride loop over the device-side count, so the
+// host needs no copy-back of the count before launching.
+template <class Eng>
+__global__ __launch_bounds__(kBlock) void queue_pass(ap_view v, ap_outputs o,
+                                                    typename Eng::Data d) {
+    LaneStats st;
+    const int n = *o.queue_count;
+    const int stride = gridDim.x * blockDim.x;
+    const int rounds = (n + stride - 1) / stride;
+    for (int r = 0; r < rounds; ++r) {
+        int q = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        if (q < n) {
+            int p = o.queue[q];
+            process_pixel<Eng>(v, o, d, p % v.width, p / v.width, true, st);
+        }
+    }
+    flush_stats(o.stats, st.iters, st.plain, st.bla, st.samples);
+}
+
+template <class Eng>
+inline hipError_t launch_pass(const ap_view& v, const ap_outputs& o,
+                              const typename Eng::Data& d, hipStream_t s) {
+    if (o.queue) {
+        int blocks = 1024;  // 4 blocks per CU; work per lane varies anyway
+        hipLaunchKernelGGL(queue_pass<Eng>, dim3(blocks), dim3(kBlock), 0, s, v, o, d);
+    } else {
+        dim3 grid((v.width + kTile - 1) / kTile, (v.height + kTile - 1) / kTile);
+        hipLaunchKernelGGL(tile_pass<Eng>, grid, dim3(kBlock), 0, s, v, o, d);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace ap

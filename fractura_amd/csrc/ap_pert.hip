@@ -1,0 +1,668 @@
+// Perturbation engines over a host-computed reference orbit Z_k.
+//
+//  PERT64  FP64 delta (w >= 1e-300): restates cpu_compute_perturbation_sample
+//          (math_core/cpu_kernels.py:373-602) with the chained-BLA lookup.
+//  FEXP64  FloatExp, FP64 mantissa + int32 exponent (w < 1e-300): restates
+//          cpu_compute_perturbation_sample_floatexp (:605-1021).
+//  FEXP32  FloatExp with an FP32 mantissa (no reference counterpart).  FP64
+//          stays where stability needs it: the near-bailout test and smooth
+//          value, Stage 2, and the BLA table build.
+//
+// The FP64 engines follow the CPU restatement operation by operation
+// (contraction off).  Two lookups are reorganised without changing results:
+// the BLA level search is a binary search over radii that never grow with
+// the level (same answer as the reference's top-down scan), and sqrt(r^2)
+// is read from the table instead of being recomputed.
+#include "ap_bla.hpp"
+#include "ap_launch.hpp"
+#include "ap_pass.hpp"
+
+namespace ap {
+
+constexpr double kInv2p16 = 1.52587890625e-05;  // 2^-16
+
+// pow2_f64 of the reference (math_core/cpu_kernels.py:161-167): 2^e with
+// e >= 1023 -> DBL_MAX and e <= -1022 -> 0.
+__device__ inline double pow2_f64(int e) {
+    if (e >= 1023) return 1.7976931348623157e+308;
+    if (e <= -1022) return 0.0;
+    return ldexp(1.0, e);
+}
+
+// Python max(a, b) (returns a unless b > a; matters only for NaN).
+__device__ inline double pymax(double a, double b) { return b > a ? b : a; }
+
+// int(log(x) / ln 2) for x > 0, computed from the binary exponent: truncation
+// toward zero of log2 x.  Equal to the reference's log-based value except
+// when x lies within rounding distance of a power of two.
+__device__ inline int trunc_log2(double x) {
+    int e;
+    double m = frexp(x, &e);  // x = m 2^e, m in [0.5, 1)
+    if (x >= 1.0) return e - 1;
+    return m == 0.5 ? e - 1 : e;
+}
+
+// Pauldelbrot test in rescaled exponent space
+// (check_glitch_cancellation_floatexp, math_core/cpu_kernels.py:170-201).
+__device__ inline int glitch_floatexp(double rx, double ry, double dzx, double dzy, int eff) {
+    double r_max = pymax(fabs(rx), fabs(ry));
+    if (r_max < 1e-4) return 0;
+    double dz_max = pymax(fabs(dzx), fabs(dzy));
+    if (dz_max <= 0.0) return 0;
+    int r_exp = trunc_log2(r_max);
+    int dz_exp = trunc_log2(dz_max);
+    int diff = r_exp - (eff + dz_exp);
+    if (diff > 10 || diff < -10) return 0;
+    double sr = pow2_f64(-r_exp), sd = pow2_f64(eff - r_exp);
+    double rxs = rx * sr, rys = ry * sr;
+    double zx = rxs + dzx * sd, zy = rys + dzy * sd;
+    double m2 = zx * zx + zy * zy;
+    double rm2 = rxs * rxs + rys * rys;
+    return m2 < 1e-6 * rm2 ? 1 : 0;
+}
+
+// Keep max(|dzx|, |dzy|) inside [2^-16, 2^16] in steps of 2^16.
+__device__ inline void renorm64(double& dzx, double& dzy, int& z_exp) {
+    double mag = pymax(fabs(dzx), fabs(dzy));
+    while (mag > 65536.0 && isfinite(mag)) {
+        dzx *= kInv2p16;
+        dzy *= kInv2p16;
+        z_exp += 16;
+        mag = pymax(fabs(dzx), fabs(dzy));
+    }
+    while (mag < kInv2p16 && mag > 0.0 && isfinite(mag)) {
+        dzx *= 65536.0;
+        dzy *= 65536.0;
+        z_exp -= 16;
+        mag = pymax(fabs(dzx), fabs(dzy));
+    }
+}
+
+__device__ inline Sample nan_exit(int k) {
+    Sample r;
+    r.count = k;
+    r.s = (float)k;
+    r.smooth = (double)k;
+    r.glitch = 1;
+    r.iters = k;
+    r.plain = 0;
+    r.bla = 0;
+    return r;
+}
+
+struct PertConsts {
+    double inv_ref, inv_exact;
+};
+
+__device__ inline PertConsts pert_consts(const ap_view& v) {
+    PertConsts c;
+    c.inv_ref = c.inv_exact = v.inv_ln2;
+    if (v.fractal_type == AP_GENERAL_MANDELBROT && v.gen_n > 1) {
+        c.inv_exact = 1.0 / log((double)v.gen_n);
+        c.inv_ref = (double)(float)c.inv_exact;
+    }
+    return c;
+}
+
+// Stage 2 of every perturbation engine: the reference escaped (or ran out),
+// so the pixel continues as a direct FP64 iteration of Z_ref_len + dz.
+template <int T>
+__device__ inline Sample stage2(const ap_view& v, const PertConsts& pc, double zx, double zy,
+                                double cx, double cy, int k, int glitch, int plain, int bla) {
+    const int max_iter = v.max_iter;
+    const int n = v.gen_n;
+    while (k < max_iter) {
+        double zx2 = zx * zx, zy2 = zy * zy;
+        double mag2 = zx2 + zy2;
+        if (isnan(mag2) || isinf(mag2)) {
+            Sample r = nan_exit(k);
+            r.plain = plain;
+            r.bla = bla;
+            return r;
+        }
+        if (mag2 > kBailout2) {
+            Sample r = escaped(k, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
+            r.glitch = glitch;
+            r.plain = plain;
+            r.bla = bla;
+            return r;
+        }
+        if (T == 0 || T == 2) {
+            zy = 2.0 * zx * zy + cy;
+            zx = zx2 - zy2 + cx;
+        } else if (T == 1 || T == 3) {
+            zy = 2.0 * fabs(zx) * fabs(zy) - cy;
+            zx = zx2 - zy2 + cx;
+        } else {
+            double pr = zx, pi = zy;
+            for (int e = 0; e < n - 1; ++e) {
+                double t = pr * zx - pi * zy;
+                pi = pr * zy + pi * zx;
+                pr = t;
+            }
+            double kzr = v.gen_kr * zx - v.gen_ki * zy;
+            double kzi = v.gen_kr * zy + v.gen_ki * zx;
+            zx = pr + kzr + cx;
+            zy = pi + kzi + cy;
+        }
+        ++k;
+        ++plain;
+    }
+    Sample r = interior(max_iter, max_iter);
+    r.glitch = glitch;
+    r.plain = plain;
+    r.bla = bla;
+    return r;
+}
+
+// =================================================================== PERT64
+struct PertData {
+    const double* zr;
+    const double* zi;
+    BlaView bla;
+};
+
+template <int T>
+__device__ Sample pert64_sample(const ap_view& v, const PertData& D, double si, double sj) {
+    const bool julia = (T == 2 || T == 3);
+    const double dcy = (v.ref_py - sj) * v.dy;
+    const double dcx = (si - v.ref_px) * v.dx;
+    double dzx = julia ? dcx : 0.0, dzy = julia ? dcy : 0.0;
+    const double dpx = julia ? 0.0 : dcx, dpy = julia ? 0.0 : dcy;
+    const int ref_len = v.ref_len;
+    const bool strict = v.strict_glitch > 0;
+    const bool has_bla = D.bla.levels > 1 && (T == 0 || T == 2 || T == 4);
+    const PertConsts pc = pert_consts(v);
+    int k = 0, glitch = 0, plain = 0, nbla = 0;
+    const int n = v.gen_n;
+    const double kr = v.gen_kr, ki = v.gen_ki;
+
+    while (k < ref_len) {
+        const double rx = D.zr[k], ry = D.zi[k];
+        const double zx = rx + dzx, zy = ry + dzy;
+        const double mag2 = zx * zx + zy * zy;
+        if (mag2 > kBailout2) {
+            if (strict && k > 15 && (rx * rx + ry * ry) < 4.0) glitch = 1;
+            Sample r = escaped(k, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
+            r.glitch = glitch;
+            r.plain = plain;
+            r.bla = nbla;
+            return r;
+        }
+        if (strict && k > 15) {
+            double r_mag2 = rx * rx + ry * ry;
+            if (mag2 < 1e-6 * r_mag2 && r_mag2 > 1e-8) glitch = 1;
+        }
+        if (has_bla && k < ref_len - 1) {
+            const double dz_mag2 = dzx * dzx + dzy * dzy;
+            const size_t row = (size_t)k * D.bla.levels;
+            if (dz_mag2 < D.bla.r2[row]) {
+                int max_d = D.bla.levels - 1;
+                if (dz_mag2 == 0.0 && (dzx != 0.0 || dzy != 0.0)) max_d = max_d < 8 ? max_d : 8;
+                int hi = min(max_d, level_limit_by_length(k, ref_len));
+                int best = highest_valid_level(hi, [&](int d) { return dz_mag2 < D.bla.r2[row + d]; });
+                double4 c = D.bla.coef[row + best];
+                double ndx = (c.x * dzx - c.y * dzy) + (c.z * dpx - c.w * dpy);
+                dzy = (c.x * dzy + c.y * dzx) + (c.z * dpy + c.w * dpx);
+                dzx = ndx;
+                ++nbla;
+                if (isnan(dzx) || isnan(dzy) || isinf(dzx) || isinf(dzy)) {
+                    Sample r = nan_exit(k);
+                    r.plain = plain;
+                    r.bla = nbla;
+                    return r;
+                }
+                k += 1 << best;
+                continue;
+            }
+        }
+        if (T == 0 || T == 2) {
+            double ndx = (2.0 * rx + dzx) * dzx - (2.0 * ry + dzy) * dzy + dpx;
+            dzy = 2.0 * ((rx + dzx) * dzy + ry * dzx) + dpy;
+            dzx = ndx;
+        } else if (T == 1 || T == 3) {
+            double ndx = (2.0 * rx + dzx) * dzx - (2.0 * ry + dzy) * dzy + dpx;
+            double sxs = rx >= 0.0 ? 1.0 : -1.0;
+            double sys = ry >= 0.0 ? 1.0 : -1.0;
+            double ddx = (rx + dzx) * sxs >= 0.0 ? sxs * dzx : -sxs * (2.0 * rx + dzx);
+            double ddy = (ry + dzy) * sys >= 0.0 ? sys * dzy : -sys * (2.0 * ry + dzy);
+            dzy = 2.0 * (fabs(rx) * ddy + fabs(ry) * ddx + ddx * ddy) - dpy;
+            dzx = ndx;
+        } else {
+            // (Z+dz)^n - Z^n by the binomial recurrence of the reference.
+            double pr = dzx, pi = dzy, qr = rx, qi = ry;
+            for (int e = 0; e < n - 1; ++e) {
+                double t1r = pr * rx - pi * ry, t1i = pr * ry + pi * rx;
+                double sr = pr + qr, sim = pi + qi;
+                double t2r = sr * dzx - sim * dzy, t2i = sr * dzy + sim * dzx;
+                pr = t1r + t2r;
+                pi = t1i + t2i;
+                double nq = qr * rx - qi * ry;
+                qi = qr * ry + qi * rx;
+                qr = nq;
+            }
+            double kdr = kr * dzx - ki * dzy, kdi = kr * dzy + ki * dzx;
+            dzx = pr + kdr + dpx;
+            dzy = pi + kdi + dpy;
+        }
+        ++k;
+        ++plain;
+    }
+    const double zx = D.zr[ref_len] + dzx, zy = D.zi[ref_len] + dzy;
+    const double cx = julia ? v.julia_cx : v.ref_cx + dcx;
+    const double cy = julia ? v.julia_cy : v.ref_cy + dcy;
+    if (ref_len < v.max_iter) glitch = 1;
+    return stage2<T>(v, pc, zx, zy, cx, cy, k, glitch, plain, nbla);
+}
+
+template <int T>
+struct EngPert64 {
+    using Data = PertData;
+    static constexpr bool kIterInSubsamples = true;
+    __device__ static Sample sample(const ap_view& v, const Data& d, double si, double sj) {
+        return pert64_sample<T>(v, d, si, sj);
+    }
+};
+
+// =================================================================== FEXP64
+template <int T>
+__device__ Sample fexp64_sample(const ap_view& v, const PertData& D, double si, double sj) {
+    const bool julia = (T == 2 || T == 3);
+    const double dcy = (v.ref_py - sj) * v.dy;  // normalised: dc * 2^E
+    const double dcx = (si - v.ref_px) * v.dx;
+    double dzx = julia ? dcx : 0.0, dzy = julia ? dcy : 0.0;
+    const double dpx = julia ? 0.0 : dcx, dpy = julia ? 0.0 : dcy;
+    int z_exp = 0;
+    const int E = v.e_scale;
+    const int ref_len = v.ref_len;
+    const bool strict = v.strict_glitch > 0;
+    const bool has_bla = D.bla.levels > 1 && (T == 0 || T == 2 || T == 4);
+    const PertConsts pc = pert_consts(v);
+    int k = 0, glitch = 0, plain = 0, nbla = 0;
+    const int n = v.gen_n;
+    const double kr = v.gen_kr, ki = v.gen_ki;
+
+    while (k < ref_len) {
+        const double rx = D.zr[k], ry = D.zi[k];
+        const int eff = z_exp - E;
+        if (eff >= -50) {
+            double sc = pow2_f64(eff);
+            double zx = rx + dzx * sc, zy = ry + dzy * sc;
+            double mag2 = zx * zx + zy * zy;
+            if (mag2 > kBailout2) {
+                Sample r = escaped(k, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
+                r.glitch = glitch;
+                r.plain = plain;
+                r.bla = nbla;
+                return r;
+            }
+        }
+        // With max(|dz_m|) <= 2^16 and r_max >= 1e-4 the test can only fire
+        // when eff + log2|dz_m| >= -24: below eff = -40 it returns 0.
+        if (strict && k > 15 && eff >= -40) {
+            if (glitch_floatexp(rx, ry, dzx, dzy, eff)) glitch = 1;
+        }
+        if (T == 0 || T == 2 || T == 4) {
+            if (has_bla && k < ref_len - 1) {
+                const int shift = z_exp - E;
+                const size_t row = (size_t)k * D.bla.levels;
+                bool can0;
+                double dz_mag2 = 0.0, dz_mag = 0.0, tiny = 0.0;
+                if (shift >= -510) {
+                    double sc = pow2_f64(shift);
+                    dz_mag2 = (dzx * dzx + dzy * dzy) * (sc * sc);
+                    can0 = dz_mag2 < D.bla.r2[row];
+                } else {
+                    dz_mag = sqrt(dzx * dzx + dzy * dzy);
+                    tiny = dz_mag * pow2_f64(shift > -1020 ? shift : -1020);
+                    can0 = D.bla.r2[row] > 0.0 && tiny < D.bla.r1[row];
+                }
+                if (can0) {
+                    int hi = min(D.bla.levels - 1, level_limit_by_length(k, ref_len));
+                    int best;
+                    if (shift >= -510)
+                        best = highest_valid_level(hi, [&](int d) { return dz_mag2 < D.bla.r2[row + d]; });
+                    else
+                        best = highest_valid_level(hi, [&](int d) {
+                            return D.bla.r2[row + d] > 0.0 && tiny < D.bla.r1[row + d];
+                        });
+                    double4 c = D.bla.coef[row + best];
+                    double scale_c = (z_exp >= -1020 && z_exp <= 1020) ? pow2_f64(-z_exp) : 0.0;
+                    double dxs = dpx * scale_c, dys = dpy * scale_c;
+                    double ndx = (c.x * dzx - c.y * dzy) + (c.z * dxs - c.w * dys);
+                    dzy = (c.x * dzy + c.y * dzx) + (c.z * dys + c.w * dxs);
+                    dzx = ndx;
+                    ++nbla;
+                    if (isnan(dzx) || isnan(dzy) || isinf(dzx) || isinf(dzy)) {
+                        Sample r = nan_exit(k);
+                        r.plain = plain;
+                        r.bla = nbla;
+                        return r;
+                    }
+                    renorm64(dzx, dzy, z_exp);
+                    k += 1 << best;
+                    continue;
+                }
+            }
+        }
+        if (T == 0 || T == 2) {
+            double lx = 2.0 * (rx * dzx - ry * dzy);
+            double ly = 2.0 * (rx * dzy + ry * dzx);
+            const int snl = z_exp - E;
+            if (snl >= -1020) {
+                double s = pow2_f64(snl);
+                lx += (dzx * dzx - dzy * dzy) * s;
+                ly += (2.0 * dzx * dzy) * s;
+            }
+            if (z_exp >= -1020 && z_exp <= 1020) {
+                double s = pow2_f64(-z_exp);
+                lx += dpx * s;
+                ly += dpy * s;
+            }
+            dzx = lx;
+            dzy = ly;
+            renorm64(dzx, dzy, z_exp);
+        } else if (T == 1 || T == 3) {
+            const int snl = z_exp - E;
+            double s_nl = snl >= -1020 ? pow2_f64(snl) : 0.0;
+            double s_c = (z_exp >= -1020 && z_exp <= 1020) ? pow2_f64(-z_exp) : 0.0;
+            double sxs = rx >= 0.0 ? 1.0 : -1.0;
+            double sys = ry >= 0.0 ? 1.0 : -1.0;
+            double s_esc = eff >= -1020 ? pow2_f64(eff) : 0.0;
+            double adx = dzx * s_esc, ady = dzy * s_esc;
+            double fx, ex, fy, ey;
+            if ((rx + adx) * sxs >= 0.0) {
+                fx = sxs;
+                ex = 0.0;
+            } else {
+                fx = -sxs;
+                ex = eff <= 1020 ? -sxs * 2.0 * rx * pow2_f64(-eff) : 0.0;
+            }
+            if ((ry + ady) * sys >= 0.0) {
+                fy = sys;
+                ey = 0.0;
+            } else {
+                fy = -sys;
+                ey = eff <= 1020 ? -sys * 2.0 * ry * pow2_f64(-eff) : 0.0;
+            }
+            double lx = (2.0 * rx + dzx * s_nl) * dzx - (2.0 * ry + dzy * s_nl) * dzy + dpx * s_c;
+            double dxn = fx * dzx + ex;
+            double dyn = fy * dzy + ey;
+            double ly = 2.0 * (fabs(rx) * dyn + fabs(ry) * dxn + dxn * dyn * s_nl) - dpy * s_c;
+            dzx = lx;
+            dzy = ly;
+            // The Burning Ship branch of the reference renormalises without a
+            // finiteness guard (math_core/cpu_kernels.py:868-878), which never
+            // ends on an infinite delta; the guard changes nothing else.
+            renorm64(dzx, dzy, z_exp);
+        } else {
+            double pr = dzx, pi = dzy, qr = rx, qi = ry;
+            const int snl = z_exp - E;
+            double s_nl = snl >= -1020 ? pow2_f64(snl) : 0.0;
+            for (int e = 0; e < n - 1; ++e) {
+                double t1r = pr * rx - pi * ry, t1i = pr * ry + pi * rx;
+                double t2r = qr * dzx - qi * dzy, t2i = qr * dzy + qi * dzx;
+                double t3r = pr * dzx - pi * dzy, t3i = pr * dzy + pi * dzx;
+                pr = t1r + t2r + t3r * s_nl;
+                pi = t1i + t2i + t3i * s_nl;
+                double nq = qr * rx - qi * ry;
+                qi = qr * ry + qi * rx;
+                qr = nq;
+            }
+            double kdr = kr * dzx - ki * dzy, kdi = kr * dzy + ki * dzx;
+            double s_c = (z_exp >= -1020 && z_exp <= 1020) ? pow2_f64(-z_exp) : 0.0;
+            dzx = pr + kdr + dpx * s_c;
+            dzy = pi + kdi + dpy * s_c;
+            renorm64(dzx, dzy, z_exp);
+        }
+        ++k;
+        ++plain;
+    }
+    const int eff = z_exp - E;
+    const double sfb = eff >= -60 ? pow2_f64(eff) : 0.0;
+    const double zx = D.zr[ref_len] + dzx * sfb, zy = D.zi[ref_len] + dzy * sfb;
+    const double scfb = E <= 1020 ? pow2_f64(-E) : 0.0;
+    const double cx = julia ? v.julia_cx : v.ref_cx + dpx * scfb;
+    const double cy = julia ? v.julia_cy : v.ref_cy + dpy * scfb;
+    const double mag2 = zx * zx + zy * zy;
+    if (mag2 > kBailout2) {
+        Sample r = escaped(ref_len, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
+        r.glitch = glitch;
+        r.plain = plain;
+        r.bla = nbla;
+        return r;
+    }
+    if (ref_len < v.max_iter) glitch = 1;
+    return stage2<T>(v, pc, zx, zy, cx, cy, k, glitch, plain, nbla);
+}
+
+template <int T>
+struct EngFexp64 {
+    using Data = PertData;
+    static constexpr bool kIterInSubsamples = true;
+    __device__ static Sample sample(const ap_view& v, const Data& d, double si, double sj) {
+        return fexp64_sample<T>(v, d, si, sj);
+    }
+};
+
+// =================================================================== FEXP32
+// State: float mantissa pair (dzx, dzy) and int exponent z_exp, actual delta
+// = dz_m * 2^(z_exp - E).  Z_k is read as float2 from an FP32 copy of the
+// orbit.  The mantissa is renormalised to max|dz_m| in [1, 2) by its binary
+// exponent whenever it leaves [2^-32, 2^32] (FP32 has the range to wait).
+// BLA: A and B are float32 mantissas with int exponents; validity compares
+// log2|dz| with log2 r in float32.
+struct Fexp32Data {
+    const double* zr;
+    const double* zi;
+    const float2* zf;
+    BlaView bla;
+};
+
+__device__ inline void renorm32(float& dzx, float& dzy, int& z_exp) {
+    float m = fmaxf(fabsf(dzx), fabsf(dzy));
+    if ((m > 4294967296.0f || m < 2.3283064365386963e-10f) && m > 0.0f && isfinite(m)) {
+        int e = ilogbf(m);
+        dzx = ldexpf(dzx, -e);
+        dzy = ldexpf(dzy, -e);
+        z_exp += e;
+    }
+}
+
+template <int T>
+__device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si, double sj) {
+    const bool julia = (T == 2 || T == 3);
+    const double dcy_d = (v.ref_py - sj) * v.dy;
+    const double dcx_d = (si - v.ref_px) * v.dx;
+    const float dcx = (float)dcx_d, dcy = (float)dcy_d;
+    float dzx = julia ? dcx : 0.0f, dzy = julia ? dcy : 0.0f;
+    const float dpx = julia ? 0.0f : dcx, dpy = julia ? 0.0f : dcy;
+    int z_exp = 0;
+    renorm32(dzx, dzy, z_exp);
+    const int E = v.e_scale;
+    const int ref_len = v.ref_len;
+    const bool strict = v.strict_glitch > 0;
+    const bool has_bla = D.bla.levels > 1 && (T == 0 || T == 2 || T == 4);
+    const PertConsts pc = pert_consts(v);
+    int k = 0, glitch = 0, plain = 0, nbla = 0;
+    const int n = v.gen_n;
+    const float kr = (float)v.gen_kr, ki = (float)v.gen_ki;
+
+    while (k < ref_len) {
+        const int eff = z_exp - E;
+        if (eff >= -50) {
+            // Near bailout: the test and the smooth value in FP64.
+            double sc = ldexp(1.0, eff);
+            double zx = D.zr[k] + (double)dzx * sc, zy = D.zi[k] + (double)dzy * sc;
+            double mag2 = zx * zx + zy * zy;
+            if (mag2 > kBailout2) {
+                Sample r = escaped(k, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
+                r.glitch = glitch;
+                r.plain = plain;
+                r.bla = nbla;
+                return r;
+            }
+        }
+        const float2 Z = D.zf[k];
+        if (strict && k > 15 && eff >= -40) {
+            // |Z + dz|^2 < 1e-6 |Z|^2 in actual units, for |Z| >= 1e-4.
+            float rmax = fmaxf(fabsf(Z.x), fabsf(Z.y));
+            if (rmax >= 1e-4f && (dzx != 0.0f || dzy != 0.0f)) {
+                float zx = Z.x + ldexpf(dzx, eff), zy = Z.y + ldexpf(dzy, eff);
+                float m2 = zx * zx + zy * zy;
+                float rm2 = Z.x * Z.x + Z.y * Z.y;
+                if (m2 < 1e-6f * rm2) glitch = 1;
+            }
+        }
+        if (T == 0 || T == 2 || T == 4) {
+            if (has_bla && k < ref_len - 1) {
+                const size_t row = (size_t)k * D.bla.levels;
+                // log2 |dz| in actual units.
+                float m2 = dzx * dzx + dzy * dzy;
+                float ldz = m2 > 0.0f ? 0.5f * __log2f(m2) + (float)(z_exp - E) : -INFINITY;
+                if (ldz < D.bla.lr[row]) {
+                    int hi = min(D.bla.levels - 1, level_limit_by_length(k, ref_len));
+                    int best = highest_valid_level(hi, [&](int d) { return ldz < D.bla.lr[row + d]; });
+                    const Coef32 c = D.bla.c32[row + best];
+                    // dz' = A dz + B dc: exponents e1 = z_exp + ae (A dz) and
+                    // e2 = be (B dc, dc in units of 2^-E); the larger one becomes
+                    // the new z_exp and the other term is shifted down.
+                    float tr = c.ar * dzx - c.ai * dzy, ti = c.ar * dzy + c.ai * dzx;
+                    float ur = c.br * dpx - c.bi * dpy, ui = c.br * dpy + c.bi * dpx;
+                    int e1 = z_exp + c.ae, e2 = c.be;
+                    if (tr == 0.0f && ti == 0.0f) e1 = e2;
+                    if (ur == 0.0f && ui == 0.0f) e2 = e1;
+                    int ne = max(e1, e2);
+                    dzx = ldexpf(tr, e1 - ne) + ldexpf(ur, e2 - ne);
+                    dzy = ldexpf(ti, e1 - ne) + ldexpf(ui, e2 - ne);
+                    z_exp = ne;
+                    ++nbla;
+                    if (!isfinite(dzx) || !isfinite(dzy)) {
+                        Sample r = nan_exit(k);
+                        r.plain = plain;
+                        r.bla = nbla;
+                        return r;
+                    }
+                    renorm32(dzx, dzy, z_exp);
+                    k += 1 << best;
+                    continue;
+                }
+            }
+        }
+        const float s_nl = ldexpf(1.0f, max(z_exp - E, -200));
+        const float cxs = ldexpf(dpx, max(-z_exp, -200)), cys = ldexpf(dpy, max(-z_exp, -200));
+        if (T == 0 || T == 2) {
+            // 2 Z dz + dz^2 2^(z_exp-E) + dc 2^-z_exp
+            float tx = dzx * s_nl, ty = dzy * s_nl;
+            float lx = 2.0f * (Z.x * dzx - Z.y * dzy) + (tx * dzx - ty * dzy) + cxs;
+            float ly = 2.0f * (Z.x * dzy + Z.y * dzx) + 2.0f * tx * dzy + cys;
+            dzx = lx;
+            dzy = ly;
+        } else if (T == 1 || T == 3) {
+            float sxs = Z.x >= 0.0f ? 1.0f : -1.0f;
+            float sys = Z.y >= 0.0f ? 1.0f : -1.0f;
+            float ax = ldexpf(dzx, max(z_exp - E, -200)), ay = ldexpf(dzy, max(z_exp - E, -200));
+            float back = ldexpf(1.0f, min(E - z_exp, 200));
+            float ddx = (Z.x + ax) * sxs >= 0.0f ? sxs * dzx : -sxs * (2.0f * Z.x * back + dzx);
+            float ddy = (Z.y + ay) * sys >= 0.0f ? sys * dzy : -sys * (2.0f * Z.y * back + dzy);
+            float lx = (2.0f * Z.x + dzx * s_nl) * dzx - (2.0f * Z.y + dzy * s_nl) * dzy + cxs;
+            float ly = 2.0f * (fabsf(Z.x) * ddy + fabsf(Z.y) * ddx + ddx * ddy * s_nl) - cys;
+            dzx = lx;
+            dzy = ly;
+        } else {
+            float pr = dzx, pi = dzy, qr = Z.x, qi = Z.y;
+            for (int e = 0; e < n - 1; ++e) {
+                float t1r = pr * Z.x - pi * Z.y, t1i = pr * Z.y + pi * Z.x;
+                float t2r = qr * dzx - qi * dzy, t2i = qr * dzy + qi * dzx;
+                float t3r = pr * dzx - pi * dzy, t3i = pr * dzy + pi * dzx;
+                pr = t1r + t2r + t3r * s_nl;
+                pi = t1i + t2i + t3i * s_nl;
+                float nq = qr * Z.x - qi * Z.y;
+                qi = qr * Z.y + qi * Z.x;
+                qr = nq;
+            }
+            dzx = pr + (kr * dzx - ki * dzy) + cxs;
+            dzy = pi + (kr * dzy + ki * dzx) + cys;
+        }
+        renorm32(dzx, dzy, z_exp);
+        ++k;
+        ++plain;
+    }
+    const int eff = z_exp - E;
+    const double sfb = eff >= -60 ? ldexp(1.0, eff) : 0.0;
+    const double zx = D.zr[ref_len] + (double)dzx * sfb, zy = D.zi[ref_len] + (double)dzy * sfb;
+    const double scfb = E <= 1020 ? pow2_f64(-E) : 0.0;
+    const double cx = julia ? v.julia_cx : v.ref_cx + dcx_d * scfb;
+    const double cy = julia ? v.julia_cy : v.ref_cy + dcy_d * scfb;
+    const double mag2 = zx * zx + zy * zy;
+    if (mag2 > kBailout2) {
+        Sample r = escaped(ref_len, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
+        r.glitch = glitch;
+        r.plain = plain;
+        r.bla = nbla;
+        return r;
+    }
+    if (ref_len < v.max_iter) glitch = 1;
+    return stage2<T>(v, pc, zx, zy, cx, cy, k, glitch, plain, nbla);
+}
+
+template <int T>
+struct EngFexp32 {
+    using Data = Fexp32Data;
+    static constexpr bool kIterInSubsamples = true;
+    __device__ static Sample sample(const ap_view& v, const Data& d, double si, double sj) {
+        return fexp32_sample<T>(v, d, si, sj);
+    }
+};
+
+// ------------------------------------------------------------- launchers --
+__global__ __launch_bounds__(256) void orbit_to_f32_kernel(const double* __restrict__ zr,
+                                                           const double* __restrict__ zi, int n,
+                                                           float2* __restrict__ out) {
+    int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n) out[k] = make_float2((float)zr[k], (float)zi[k]);
+}
+
+template <template <int> class E, class D>
+static hipError_t dispatch_pert(const ap_view& v, const ap_outputs& o, const D& d, hipStream_t s) {
+    switch (v.fractal_type) {
+    case 0: return launch_pass<E<0>>(v, o, d, s);
+    case 1: return launch_pass<E<1>>(v, o, d, s);
+    case 2: return launch_pass<E<2>>(v, o, d, s);
+    case 3: return launch_pass<E<3>>(v, o, d, s);
+    case 4: return launch_pass<E<4>>(v, o, d, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_pert64(const ap_view& v, const ap_orbit& orb, const void* bla,
+                         const ap_outputs& o, hipStream_t s) {
+    PertData d;
+    d.zr = orb.re;
+    d.zi = orb.im;
+    d.bla = bla_view(v.bla_levels > 1 ? bla : nullptr, orb.len, v.bla_levels);
+    return dispatch_pert<EngPert64>(v, o, d, s);
+}
+
+hipError_t launch_fexp(int mant, const ap_view& v, const ap_orbit& orb, const void* bla,
+                       const ap_outputs& o, void* scratch_f32_orbit, hipStream_t s) {
+    BlaView b = bla_view(v.bla_levels > 1 ? bla : nullptr, orb.len, v.bla_levels);
+    if (mant == 64) {
+        PertData d;
+        d.zr = orb.re;
+        d.zi = orb.im;
+        d.bla = b;
+        return dispatch_pert<EngFexp64>(v, o, d, s);
+    }
+    Fexp32Data d;
+    d.zr = orb.re;
+    d.zi = orb.im;
+    d.zf = (const float2*)scratch_f32_orbit;
+    d.bla = b;
+    int n = orb.len + 1;
+    hipLaunchKernelGGL(orbit_to_f32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, orb.re, orb.im,
+                       n, (float2*)scratch_f32_orbit);
+    return dispatch_pert<EngFexp32>(v, o, d, s);
+}
+
+}  // namespace ap

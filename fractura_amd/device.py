@@ -1,0 +1,157 @@
+"""Device context, streams and the pooled scratch buffers of the render path.
+
+Replaces the reference's CUDA device selection (math_core/cuda_kernels.py:
+11-30), stream-slot registry and scratch pool (math_core/engine.py:94-188).
+Device memory is allocated through torch (its caching allocator); kernels
+run in libapeiron_hip.so on torch's current stream, or on a stream the caller
+passes.  There is no CPU fallback: without a HIP device every call raises.
+"""
+import ctypes as C
+import threading
+
+import numpy as np
+
+from . import _abi
+
+_CTX = {}
+_CTX_LOCK = threading.Lock()
+
+_POOL = {}
+_POOL_LOCK = threading.Lock()
+
+_SLOTS = {}
+_SLOT_LOCK = threading.Lock()
+_NEXT_SLOT = [1]
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("no HIP device visible: the fractura_amd render path runs on the GPU only")
+    return torch
+
+
+class Device:
+    """One libapeiron_hip context on one device (calls serialised by a lock)."""
+
+    def __init__(self, index=0):
+        self.index = int(index)
+        self.lib = _abi.load()
+        h = C.c_void_p()
+        torch = _torch()
+        torch.cuda.set_device(self.index)
+        st = self.lib.ap_init(self.index, C.byref(h))
+        if st != _abi.AP_OK:
+            raise _abi.ApError(st, "ap_init", f"device {self.index}")
+        self.handle = h
+        self.lock = threading.RLock()
+        self.torch_device = torch.device("cuda", self.index)
+        cus, clk = C.c_int(), C.c_int()
+        name = C.create_string_buffer(256)
+        self.lib.ap_device_info(self.handle, C.byref(cus), C.byref(clk), name, 256)
+        self.num_cus, self.clock_khz, self.arch = cus.value, clk.value, name.value.decode()
+
+    def call(self, name, *args):
+        st = getattr(self.lib, name)(self.handle, *args)
+        if st != _abi.AP_OK:
+            raise _abi.ApError(st, name, self.lib.ap_last_error(self.handle).decode(errors="replace"))
+
+    def set_stream(self, stream=None):
+        """Launch subsequent kernels on `stream` (a torch stream, a raw handle,
+        or None/0 for torch's current stream on this device)."""
+        torch = _torch()
+        if stream is None or (isinstance(stream, int) and stream == 0):
+            handle = torch.cuda.current_stream(self.index).cuda_stream
+        elif isinstance(stream, int):
+            handle = stream
+        else:
+            handle = stream.cuda_stream
+        st = self.lib.ap_set_stream(self.handle, C.c_void_p(handle))
+        if st != _abi.AP_OK:
+            raise _abi.ApError(st, "ap_set_stream", "")
+        return handle
+
+
+def get_device(index=0):
+    with _CTX_LOCK:
+        d = _CTX.get(index)
+        if d is None:
+            d = Device(index)
+            _CTX[index] = d
+        return d
+
+
+def get_stream_slot(stream, explicit_slot=None):
+    """Per-stream scratch slot, 1..16 round robin, 0 for the default stream
+    (math_core/engine.py:106-119)."""
+    if explicit_slot is not None and explicit_slot >= 0:
+        return int(explicit_slot)
+    if stream is None or (isinstance(stream, int) and stream == 0):
+        return 0
+    with _SLOT_LOCK:
+        slot = _SLOTS.get(id(stream))
+        if slot is None:
+            slot = _NEXT_SLOT[0]
+            _NEXT_SLOT[0] = (_NEXT_SLOT[0] % 16) + 1
+            _SLOTS[id(stream)] = slot
+        return slot
+
+
+_TORCH_DTYPES = {
+    np.dtype(np.float32): "float32", np.dtype(np.float64): "float64", np.dtype(np.int32): "int32",
+    np.dtype(np.int64): "int64", np.dtype(np.uint8): "uint8", np.dtype(np.uint16): "uint16",
+}
+
+
+def torch_dtype(dtype):
+    torch = _torch()
+    name = _TORCH_DTYPES[np.dtype(dtype)]
+    return getattr(torch, name)
+
+
+def get_scratch_device_array(shape, dtype, tag="scratch", device=0):
+    """Pooled device tensor keyed by (tag, shape, dtype) (math_core/engine.py:123-134)."""
+    torch = _torch()
+    shape = (shape,) if isinstance(shape, int) else tuple(int(s) for s in shape)
+    key = (tag, shape, np.dtype(dtype).str, device)
+    with _POOL_LOCK:
+        buf = _POOL.get(key)
+        if buf is None:
+            buf = torch.empty(shape, dtype=torch_dtype(dtype), device=f"cuda:{device}")
+            _POOL[key] = buf
+        return buf
+
+
+def get_scratch_bytes(nbytes, tag, device=0):
+    """Pooled device byte buffer with capacity >= nbytes (grows by 1.5x)."""
+    torch = _torch()
+    key = (tag, "bytes", device)
+    with _POOL_LOCK:
+        buf = _POOL.get(key)
+        if buf is None or buf.numel() < nbytes:
+            buf = torch.empty(max(int(nbytes * 1.5), 65536), dtype=torch.uint8, device=f"cuda:{device}")
+            _POOL[key] = buf
+        return buf
+
+
+def clear_scratch_buffer_pool():
+    """Drop every pooled buffer (math_core/engine.py:171-188)."""
+    with _POOL_LOCK:
+        _POOL.clear()
+    with _SLOT_LOCK:
+        _SLOTS.clear()
+        _NEXT_SLOT[0] = 1
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+            torch.cuda.empty_cache()
+    except RuntimeError:
+        pass
+
+
+def ptr(t):
+    """Device address of a tensor (None -> NULL)."""
+    return None if t is None else C.c_void_p(t.data_ptr())

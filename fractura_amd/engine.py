@@ -1,0 +1,553 @@
+"""Render-call surface of the escape-time path (math_core/engine.py).
+
+`render_frame`, `render_frame_ssaa` and `recolor_frame` keep the reference's
+names, arguments and defaults (math_core/engine.py:1326-1346, 2138) and its
+outputs (RGB frame; float32 smooth map in `d_iter_buf`, -1 for the
+interior).  Underneath, every pass is a libapeiron_hip.so kernel:
+
+  engine choice by width      math_core/engine.py:1403-1408
+  grid constants              math_core/engine.py:1386-1421
+  reference orbit (host)      high_precision.find_optimal_reference_orbit
+  BLA table                   built on the device (ap_bla_build)
+  pass 1 / edge mask / pass 3 adaptive SSAA, pass 3 over a compacted queue
+  histogram colouring         LDS histogram + CDF scan + colourise
+  Pauldelbrot rebase          up to 4 passes over a device-compacted queue
+
+`render_view` is the lower-level call that returns all device outputs
+(int32 counts, float64 smooth values, the float32 map, glitch flags, RGB)
+and the iteration statistics of the frame.
+
+Additions over the reference surface (keyword-only, defaults keep the
+reference behaviour): `fexp_mantissa` (64 = reference FloatExp, 32 = the
+FP32-mantissa loop), `force_floatexp` (FloatExp above 1e-300), `device`.
+"""
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from decimal import Decimal
+
+import numpy as np
+
+from . import _abi
+from .color import get_gpu_cmap_lut
+from .config import (BLA_LEVELS, GLOBAL_MAX_ITER, HISTOGRAM_BINS, HISTOGRAM_SCHEME, INV_LN2_F32,
+                     LN_BAILOUT_LOG, RenderParameters, compute_floatexp_scale,
+                     compute_required_prec_bits, decimal_context)
+from .device import (clear_scratch_buffer_pool, get_device, get_scratch_bytes,
+                     get_scratch_device_array, get_stream_slot, ptr)
+from .high_precision import (cache_reference_orbit, compute_single_highprec_orbit,
+                             find_optimal_reference_orbit, to_dd)
+
+__all__ = [
+    "RenderParameters", "render_frame", "render_frame_ssaa", "render_frame_cpu", "recolor_frame",
+    "render_view", "compute_dynamic_max_iter", "clear_scratch_buffer_pool",
+    "get_scratch_device_array", "select_engine", "plan_view",
+]
+
+ENGINES = ("fp64", "dd", "pert64", "fexp64", "fexp32")
+
+
+def _glitch_flag(glitch_mode):
+    g = str(glitch_mode)
+    return 1 if ("Auto Rebasing" in g or "Strict" in g or "Pauldelbrot" in g) else 0
+
+
+def select_engine(plot_width_dec, precision_mode="1e-300 (Perturbation)", force_floatexp=False,
+                  fexp_mantissa=64):
+    """FP64 for w >= 1e-12, DD for 1e-24 <= w < 1e-12 when the mode names DD,
+    FloatExp below 1e-300, FP64-delta perturbation otherwise
+    (math_core/engine.py:1403-1408)."""
+    w = Decimal(str(plot_width_dec))
+    use_fexp = w < Decimal("1e-300") or force_floatexp
+    if use_fexp:
+        return "fexp32" if int(fexp_mantissa) == 32 or "fp32" in str(precision_mode).lower() else "fexp64"
+    if w >= Decimal("1e-24") and "dd" in str(precision_mode).lower():
+        return "dd"
+    if w >= Decimal("1e-12"):
+        return "fp64"
+    return "pert64"
+
+
+@dataclass
+class ViewPlan:
+    engine: str
+    view: _abi.ApView
+    center_x_dec: Decimal
+    center_y_dec: Decimal
+    plot_width_dec: Decimal
+    plot_height_dec: Decimal
+    bla: bool = False
+    bla_tol: float = 0.0
+    orbit: tuple = None           # (ref_re, ref_im, ref_len) host arrays
+    extra: dict = field(default_factory=dict)
+
+
+def plan_view(target_w, target_h, max_iter, center_x_dec, center_y_dec, plot_width_dec,
+              ssaa_factor=1, fractal_type=0, julia_cx=0.0, julia_cy=0.0, gen_n=3, gen_kr=0.25,
+              gen_ki=1.0, precision_mode="1e-300 (Perturbation)", bbsa_tol=1e-4, bbsa_order=4,
+              glitch_mode="Off (Single-Ref)", color_scheme_id=0, is_hdr=False, palette_offset=0.0,
+              color_density=1.0, color_contrast=1.0, force_floatexp=False, fexp_mantissa=64,
+              compute_orbit=True, probe=True):
+    """Host-side constants of one frame: engine, grid doubles, reference orbit.
+
+    The grid follows math_core/engine.py:1386-1421: h = w*H/W in Decimal,
+    dx = float(w/(W-1))/factor, dy = float(h/(H-1))/factor,
+    x_min = float(cx) - float(w)*0.5, y_max = float(cy) + float(h)*0.5 (FP64);
+    the DD grid is to_dd of the Decimal values; FloatExp spacings are
+    float(w*2^E/(W-1))/factor.
+    """
+    center_x_dec = Decimal(str(center_x_dec))
+    center_y_dec = Decimal(str(center_y_dec))
+    plot_width_dec = Decimal(str(plot_width_dec))
+    W, H = int(target_w), int(target_h)
+    f = max(1, int(ssaa_factor))
+    engine = select_engine(plot_width_dec, precision_mode, force_floatexp, fexp_mantissa)
+    v = _abi.ApView()
+    v.width, v.height, v.factor, v.max_iter = W, H, f, int(max_iter)
+    v.fractal_type, v.gen_n = int(fractal_type), int(gen_n)
+    v.gen_kr, v.gen_ki = float(gen_kr), float(gen_ki)
+    v.inv_ln2, v.ln_bailout_log = INV_LN2_F32, LN_BAILOUT_LOG
+    v.scheme_id, v.is_hdr = int(color_scheme_id), 1 if is_hdr else 0
+    v.palette_offset, v.color_density, v.color_contrast = float(palette_offset), float(color_density), float(color_contrast)
+    v.mode = _abi.AP_MODE_CENTRE
+    with decimal_context(plot_width_dec):
+        aspect = Decimal(H) / Decimal(W)
+        plot_h = plot_width_dec * aspect
+        dx_f = float(plot_width_dec / Decimal(max(1, W - 1)))
+        dy_f = float(plot_h / Decimal(max(1, H - 1)))
+        plan = ViewPlan(engine, v, center_x_dec, center_y_dec, plot_width_dec, plot_h)
+        if engine == "fp64":
+            v.x_min = float(center_x_dec) - float(plot_width_dec) * 0.5
+            v.y_max = float(center_y_dec) + float(plot_h) * 0.5
+            v.dx, v.dy = dx_f / float(f), dy_f / float(f)
+            v.julia_cx, v.julia_cy = float(julia_cx), float(julia_cy)
+        elif engine == "dd":
+            v.x_min, v.x_min_lo = to_dd(center_x_dec - plot_width_dec / Decimal(2))
+            v.y_max, v.y_max_lo = to_dd(center_y_dec + plot_h / Decimal(2))
+            v.dx, v.dx_lo = to_dd((plot_width_dec / Decimal(max(1, W - 1))) / Decimal(f))
+            v.dy, v.dy_lo = to_dd((plot_h / Decimal(max(1, H - 1))) / Decimal(f))
+            v.julia_cx, v.julia_cx_lo = to_dd(Decimal(str(julia_cx)))
+            v.julia_cy, v.julia_cy_lo = to_dd(Decimal(str(julia_cy)))
+        else:
+            v.julia_cx, v.julia_cy = float(julia_cx), float(julia_cy)
+            v.strict_glitch = _glitch_flag(glitch_mode)
+            if engine in ("fexp64", "fexp32"):
+                E = compute_floatexp_scale(plot_width_dec)
+                sf = Decimal(2) ** E
+                nw, nh = plot_width_dec * sf, plot_h * sf
+                v.e_scale = E
+                v.dx = float(nw / Decimal(max(1, W - 1))) / float(f)
+                v.dy = float(nh / Decimal(max(1, H - 1))) / float(f)
+                # FloatExp forces chained BLA with tol <= 1e-6
+                # (math_core/engine.py:1719-1722).
+                order, tol = int(bbsa_order), float(bbsa_tol)
+                if order in (4, 8):
+                    order = 32
+                    tol = min(tol, 1e-6)
+            else:
+                v.dx, v.dy = dx_f / float(f), dy_f / float(f)
+                order, tol = int(bbsa_order), float(bbsa_tol)
+            plan.bla = order == 32 and int(fractal_type) in (0, 2, 4) and tol > 0.0
+            plan.bla_tol = tol
+            if compute_orbit:
+                cx, cy, u, vv, re, im, ref_len = find_optimal_reference_orbit(
+                    center_x_dec, center_y_dec, plot_width_dec, plot_h, int(max_iter), W, H,
+                    fractal_type=int(fractal_type), julia_cx=julia_cx, julia_cy=julia_cy,
+                    gen_n=int(gen_n), gen_kr=gen_kr, gen_ki=gen_ki,
+                    is_floatexp=engine.startswith("fexp"), probe=probe)
+                set_reference(plan, cx, cy, u, vv, re, im, ref_len)
+    return plan
+
+
+def set_reference(plan, ref_cx_dec, ref_cy_dec, u, v_frac, ref_re, ref_im, ref_len,
+                  ref_px=None, ref_py=None):
+    """Attach a reference orbit: position in subsample units
+    ref_px = (W-1)*u*factor (math_core/engine.py:1507-1508) unless given."""
+    v = plan.view
+    f = v.factor
+    v.ref_px = float(v.width - 1) * float(u) * float(f) if ref_px is None else float(ref_px)
+    v.ref_py = float(v.height - 1) * float(v_frac) * float(f) if ref_py is None else float(ref_py)
+    v.ref_cx, v.ref_cy = float(ref_cx_dec), float(ref_cy_dec)
+    v.ref_len = int(ref_len)
+    plan.orbit = (np.ascontiguousarray(ref_re[: ref_len + 1], dtype=np.float64),
+                  np.ascontiguousarray(ref_im[: ref_len + 1], dtype=np.float64), int(ref_len))
+    plan.extra["ref_cx_dec"], plan.extra["ref_cy_dec"] = ref_cx_dec, ref_cy_dec
+
+
+@dataclass
+class Frame:
+    """Device outputs of one frame (torch tensors) and its statistics."""
+
+    counts: object
+    smooth: object
+    iter_map: object
+    glitch: object
+    rgb: object
+    stats: object
+    engine: str
+    plan: ViewPlan
+    rebase_passes: int = 0
+    glitch_counts: list = field(default_factory=list)
+
+    def host_stats(self):
+        s = self.stats.cpu().numpy().view(np.uint64)
+        return {"iterations": int(s[0]), "plain_steps": int(s[1]), "bla_steps": int(s[2]),
+                "samples": int(s[3])}
+
+
+class _Orbit:
+    """Device copy of a reference orbit and its BLA table."""
+
+    def __init__(self, dev, plan, slot, tag="ref"):
+        torch = __import__("torch")
+        re, im, ref_len = plan.orbit
+        self.len = ref_len
+        nb = (ref_len + 1) * 8
+        self.re_buf = get_scratch_bytes(nb, f"{tag}_re_{slot}", dev.index)
+        self.im_buf = get_scratch_bytes(nb, f"{tag}_im_{slot}", dev.index)
+        self.re_buf[:nb].copy_(torch.from_numpy(re.view(np.uint8)), non_blocking=False)
+        self.im_buf[:nb].copy_(torch.from_numpy(im.view(np.uint8)), non_blocking=False)
+        self.abi = _abi.ApOrbit(self.re_buf.data_ptr(), self.im_buf.data_ptr(), ref_len, 0)
+        self.table = None
+        self.levels = 0
+        v = plan.view
+        if plan.bla and ref_len >= 4:
+            levels = BLA_LEVELS
+            nbytes = dev.lib.ap_bla_bytes(ref_len, levels)
+            self.table = get_scratch_bytes(nbytes, f"{tag}_bla_{slot}", dev.index)
+            dev.call("ap_bla_build", C.byref(self.abi), v.fractal_type, v.gen_n, v.gen_kr, v.gen_ki,
+                     plan.bla_tol, levels, ptr(self.table))
+            self.levels = levels
+
+    def table_ptr(self):
+        return ptr(self.table) if self.table is not None else None
+
+
+def _launch(dev, engine, view, orbit, outs):
+    if engine == "fp64":
+        dev.call("ap_render_fp64", C.byref(view), C.byref(outs))
+    elif engine == "dd":
+        dev.call("ap_render_dd", C.byref(view), C.byref(outs))
+    else:
+        view.bla_levels = orbit.levels
+        fn = {"pert64": "ap_render_pert64", "fexp64": "ap_render_fexp64",
+              "fexp32": "ap_render_fexp32"}[engine]
+        dev.call(fn, C.byref(view), C.byref(orbit.abi), orbit.table_ptr(), C.byref(outs))
+
+
+def _outs(counts=None, smooth=None, iter_map=None, glitch=None, rgb=None, lut=None, mask=None,
+          cdf=None, num_bins=0, queue=None, queue_count=None, stats=None):
+    o = _abi.ApOutputs()
+    for name, t in (("counts", counts), ("smooth", smooth), ("iter_map", iter_map),
+                    ("glitch", glitch), ("rgb", rgb), ("lut", lut), ("mask", mask), ("cdf", cdf),
+                    ("queue", queue), ("queue_count", queue_count), ("stats", stats)):
+        setattr(o, name, None if t is None else t.data_ptr())
+    o.num_bins = num_bins
+    return o
+
+
+def _select_rebase_pixel(idx, width, height, iters, max_iter, visited):
+    """Cluster-targeted rebase point (math_core/engine.py:1924-1992): among
+    the glitched pixels (flat indices `idx`, ascending), take the 16x16 bin
+    with most glitches; inside it prefer the unvisited interior pixel nearest
+    the cluster median, else the unvisited escaped pixel with the highest
+    count, else the unvisited pixel nearest the median."""
+    rows, cols = idx // width, idx % width
+    br = np.clip((rows * 16) // height, 0, 15)
+    bc = np.clip((cols * 16) // width, 0, 15)
+    b = br * 16 + bc
+    counts = np.bincount(b, minlength=256)
+    for bin_id in np.argsort(-counts, kind="stable"):
+        if counts[bin_id] == 0:
+            break
+        sel = b == bin_id
+        pr, pc = rows[sel], cols[sel]
+        it = iters[sel]
+        mr, mc = np.median(pr), np.median(pc)
+        interior = (it < 0) | (it >= max_iter)
+        if np.any(interior):
+            ir, ic = pr[interior], pc[interior]
+            for d in np.argsort((ir - mr) ** 2 + (ic - mc) ** 2, kind="stable"):
+                cand = (int(ir[d]), int(ic[d]))
+                if cand not in visited:
+                    return cand
+        esc = (it > 0) & (it < max_iter)
+        if np.any(esc):
+            er, ec, ei = pr[esc], pc[esc], it[esc]
+            for d in np.argsort(-ei, kind="stable"):
+                cand = (int(er[d]), int(ec[d]))
+                if cand not in visited:
+                    return cand
+        for d in np.argsort((pr - mr) ** 2 + (pc - mc) ** 2, kind="stable"):
+            cand = (int(pr[d]), int(pc[d]))
+            if cand not in visited:
+                return cand
+    for r, c in zip(rows, cols):
+        if (int(r), int(c)) not in visited:
+            return (int(r), int(c))
+    return None
+
+
+def render_view(plan, cmap_name="inferno", edge_threshold=0.35, glitch_mode="Off (Single-Ref)",
+                device=0, stream=None, stream_slot=None, d_iter_buf=None, d_rgb_buf=None,
+                want_counts=True, want_smooth=True, rebase=True, stats=True):
+    """Run every device pass of one frame; returns a Frame of device tensors."""
+    import torch
+
+    dev = get_device(device)
+    with dev.lock:
+        dev.set_stream(stream)
+        slot = get_stream_slot(stream, stream_slot)
+        v = plan.view.copy()
+        W, H, f = v.width, v.height, v.factor
+        n = W * H
+        engine = plan.engine
+        hist = v.scheme_id == HISTOGRAM_SCHEME
+        dt = np.uint16 if v.is_hdr else np.uint8
+        rgb = d_rgb_buf if d_rgb_buf is not None else get_scratch_device_array((H, W, 3), dt, f"out_rgb_{slot}", device)
+        it = d_iter_buf if d_iter_buf is not None else get_scratch_device_array((H, W), np.float32, f"iter_map_{slot}", device)
+        counts = get_scratch_device_array((H, W), np.int32, f"counts_{slot}", device) if want_counts else None
+        smooth = get_scratch_device_array((H, W), np.float64, f"smooth_{slot}", device) if want_smooth else None
+        pert = engine in ("pert64", "fexp64", "fexp32")
+        glitch = get_scratch_device_array((H, W), np.uint8, f"glitch_out_{slot}", device) if pert else None
+        st = torch.zeros(_abi.AP_STAT_COUNT, dtype=torch.int64, device=dev.torch_device) if stats else None
+        lut = get_gpu_cmap_lut(cmap_name, 2048, bool(v.is_hdr), dev.torch_device)
+        orbit = _Orbit(dev, plan, slot) if pert else None
+
+        # Pass 1: centre sample of every pixel (RGB later for the histogram).
+        v.mode = _abi.AP_MODE_CENTRE
+        v.rebase_pass = 0
+        o = _outs(counts, smooth, it, glitch, None if hist else rgb, lut, stats=st)
+        _launch(dev, engine, v, orbit, o)
+
+        cdf = None
+        if hist:
+            hbuf = get_scratch_device_array((HISTOGRAM_BINS,), np.int32, f"hist_bins_{slot}", device)
+            cdf = get_scratch_device_array((HISTOGRAM_BINS,), np.float32, f"cdf_lut_{slot}", device)
+            dev.call("ap_hist_build", ptr(it), n, v.max_iter, HISTOGRAM_BINS, ptr(hbuf))
+            dev.call("ap_hist_cdf", ptr(hbuf), HISTOGRAM_BINS, ptr(cdf))
+            dev.call("ap_colorize", C.byref(v), ptr(it), ptr(cdf), HISTOGRAM_BINS, ptr(lut), ptr(rgb))
+
+        if f > 1:
+            # Adaptive SSAA: edge mask -> compacted queue -> factor^2 subsamples.
+            edge = get_scratch_device_array((H, W), np.uint8, f"edge_mask_{slot}", device)
+            queue = get_scratch_device_array((n,), np.int32, f"edge_queue_{slot}", device)
+            qn = get_scratch_device_array((1,), np.int32, f"edge_count_{slot}", device)
+            dev.call("ap_edge_mask", ptr(it), W, H, C.c_float(np.float32(edge_threshold)), ptr(edge))
+            dev.call("ap_compact", ptr(edge), n, ptr(queue), ptr(qn))
+            v.mode = _abi.AP_MODE_EDGE_SUBSAMPLES
+            o = _outs(None, None, it, glitch, rgb, lut, mask=edge,
+                      cdf=cdf, num_bins=HISTOGRAM_BINS if hist else 0,
+                      queue=queue, queue_count=qn, stats=st)
+            _launch(dev, engine, v, orbit, o)
+            v.mode = _abi.AP_MODE_CENTRE
+
+        frame = Frame(counts, smooth, it, glitch, rgb, st, engine, plan)
+        allow = _glitch_flag(glitch_mode) == 1
+        if pert and allow and rebase and plan.plot_width_dec < Decimal("1e-12"):
+            _rebase_loop(dev, plan, v, frame, lut, cdf, slot, glitch_mode)
+        return frame
+
+
+def _rebase_loop(dev, plan, v, frame, lut, cdf, slot, glitch_mode):
+    """Pauldelbrot rebasing (math_core/engine.py:1890-2113): up to 4 passes; each
+    compacts the glitch mask on the device, picks a new reference pixel on the
+    host from the compacted indices, computes its orbit, rebuilds the BLA
+    table on the device and re-renders only the queued pixels."""
+    import torch
+
+    W, H, f = v.width, v.height, v.factor
+    n = W * H
+    exhaustive = "Strict" in str(glitch_mode)
+    queue = get_scratch_device_array((n,), np.int32, f"glitch_queue_{slot}", dev.index)
+    qn = get_scratch_device_array((1,), np.int32, f"glitch_count_{slot}", dev.index)
+    gin = get_scratch_device_array((H, W), np.uint8, f"glitch_in_{slot}", dev.index)
+    init_r = int(np.clip(v.ref_py / f, 0, H - 1))
+    init_c = int(np.clip(v.ref_px / f, 0, W - 1))
+    visited = {(init_r, init_c)}
+    prev, stall, passes = None, 0, 0
+    hist = v.scheme_id == HISTOGRAM_SCHEME
+    while passes < 4:
+        dev.call("ap_compact", ptr(frame.glitch), n, ptr(queue), ptr(qn))
+        count = int(qn.cpu().item())
+        frame.glitch_counts.append(count)
+        if count == 0 or (exhaustive and count <= 512):
+            break
+        if prev is not None and count >= prev:
+            stall += 1
+            if stall >= 2:
+                break
+        else:
+            stall = 0
+        prev = count
+        passes += 1
+        idx_dev = queue[:count]
+        idx = idx_dev.cpu().numpy().astype(np.int64)
+        iters = frame.iter_map.view(-1)[idx_dev.long()].cpu().numpy()
+        pick = _select_rebase_pixel(idx, W, H, iters, v.max_iter, visited)
+        if pick is None:
+            break
+        g_row, g_col = pick
+        visited.add(pick)
+        with decimal_context(plan.plot_width_dec):
+            x_min = plan.center_x_dec - plan.plot_width_dec / Decimal(2)
+            y_max = plan.center_y_dec + plan.plot_height_dec / Decimal(2)
+            rcx = x_min + (Decimal(int(g_col)) / Decimal(max(1, W - 1))) * plan.plot_width_dec
+            rcy = y_max - (Decimal(int(g_row)) / Decimal(max(1, H - 1))) * plan.plot_height_dec
+            bits = compute_required_prec_bits(plan.plot_width_dec)
+            re, im, rl = compute_single_highprec_orbit(rcx, rcy, v.max_iter, v.fractal_type,
+                                                       v.julia_cx, v.julia_cy, bits, v.gen_n,
+                                                       v.gen_kr, v.gen_ki)
+        if rl > 100:
+            cache_reference_orbit(rcx, rcy, plan.plot_width_dec, v.max_iter, re, im, rl,
+                                  v.fractal_type, v.julia_cx, v.julia_cy, v.gen_n, v.gen_kr,
+                                  v.gen_ki, plan.engine.startswith("fexp"))
+        sub = ViewPlan(plan.engine, v.copy(), plan.center_x_dec, plan.center_y_dec,
+                       plan.plot_width_dec, plan.plot_height_dec, plan.bla, plan.bla_tol)
+        set_reference(sub, rcx, rcy, 0.0, 0.0, re, im, rl,
+                      ref_px=float(g_col) * float(f), ref_py=float(g_row) * float(f))
+        orbit = _Orbit(dev, sub, slot, tag="rebase")
+        rv = sub.view
+        rv.rebase_pass = passes
+        gin.copy_(frame.glitch)
+        if hist:
+            rv.mode = _abi.AP_MODE_ALL_SUBSAMPLES
+            o = _outs(frame.counts, frame.smooth, frame.iter_map, frame.glitch, frame.rgb, lut,
+                      mask=gin, cdf=cdf, num_bins=HISTOGRAM_BINS, queue=queue, queue_count=qn,
+                      stats=frame.stats)
+        else:
+            rv.mode = _abi.AP_MODE_CENTRE if f == 1 else _abi.AP_MODE_ALL_SUBSAMPLES
+            o = _outs(frame.counts, frame.smooth, frame.iter_map, frame.glitch, frame.rgb, lut,
+                      mask=gin, queue=queue, queue_count=qn, stats=frame.stats)
+        _launch(dev, plan.engine, rv, orbit, o)
+    frame.rebase_passes = passes
+
+
+def render_frame(params):
+    """Unified entry point (math_core/engine.py:1326-1328)."""
+    return render_frame_ssaa(params)
+
+
+def render_frame_ssaa(target_w, target_h=None, max_iter=None, center_x_dec=None, center_y_dec=None,
+                      plot_width_dec=None, cmap_name=None, ssaa_factor=1, edge_threshold=0.35,
+                      color_scheme_id=0, precision_mode="1e-300 (Perturbation)", fractal_type=0,
+                      julia_cx=0.0, julia_cy=0.0, bbsa_tol=1e-4, bbsa_order=4,
+                      glitch_mode="Off (Legacy Single-Ref)", gen_n=3, gen_kr=0.25, gen_ki=1.0,
+                      d_rgb_buf=None, h_rgb_buf=None, stream=0, is_hdr=False,
+                      compute_device="GPU", cpu_threads=2, palette_offset=0.0,
+                      color_density=1.0, color_contrast=1.0, sync_stream=True, d_iter_buf=None,
+                      h_iter_buf=None, stream_slot=None, *, fexp_mantissa=64,
+                      force_floatexp=False, device=0):
+    """Render one frame and return its RGB array (H, W, 3) uint8 (uint16 if
+    is_hdr) (math_core/engine.py:1339-2135).  The float32 smooth map is left
+    in d_iter_buf (a device tensor) and copied to h_iter_buf when given."""
+    if isinstance(target_w, RenderParameters):
+        p = target_w
+        return render_frame_ssaa(
+            p.target_w, p.target_h, p.max_iter, p.center_x_dec, p.center_y_dec, p.plot_width_dec,
+            p.cmap_name, ssaa_factor=p.ssaa_factor, edge_threshold=p.edge_threshold,
+            color_scheme_id=p.color_scheme_id, precision_mode=p.precision_mode,
+            fractal_type=p.fractal_type, julia_cx=p.julia_cx, julia_cy=p.julia_cy,
+            bbsa_tol=p.bbsa_tol, bbsa_order=p.bbsa_order, glitch_mode=p.glitch_mode,
+            gen_n=p.gen_n, gen_kr=p.gen_kr, gen_ki=p.gen_ki,
+            d_rgb_buf=p.d_rgb_buf if d_rgb_buf is None else d_rgb_buf,
+            h_rgb_buf=p.h_rgb_buf if h_rgb_buf is None else h_rgb_buf,
+            stream=p.stream if p.stream is not None else stream, is_hdr=p.is_hdr,
+            compute_device=p.compute_device, cpu_threads=p.cpu_threads,
+            palette_offset=p.palette_offset, color_density=p.color_density,
+            color_contrast=p.color_contrast, sync_stream=getattr(p, "sync_stream", sync_stream),
+            d_iter_buf=getattr(p, "d_iter_buf", d_iter_buf),
+            h_iter_buf=getattr(p, "h_iter_buf", h_iter_buf),
+            stream_slot=getattr(p, "stream_slot", stream_slot),
+            fexp_mantissa=fexp_mantissa, force_floatexp=force_floatexp, device=device)
+    if compute_device != "GPU":
+        raise NotImplementedError(
+            "fractura_amd renders on the GPU only; the CPU restatement under oracle/ is a test checker")
+    plan = plan_view(target_w, target_h, max_iter, center_x_dec, center_y_dec, plot_width_dec,
+                     ssaa_factor, fractal_type, julia_cx, julia_cy, gen_n, gen_kr, gen_ki,
+                     precision_mode, bbsa_tol, bbsa_order, glitch_mode, color_scheme_id, is_hdr,
+                     palette_offset, color_density, color_contrast, force_floatexp, fexp_mantissa)
+    frame = render_view(plan, cmap_name or "inferno", edge_threshold, glitch_mode, device, stream,
+                        stream_slot, d_iter_buf, d_rgb_buf, want_counts=False, want_smooth=False)
+    out = frame.rgb.cpu().numpy()
+    if h_rgb_buf is not None and h_rgb_buf.shape == out.shape and h_rgb_buf.dtype == out.dtype:
+        h_rgb_buf[...] = out
+        out = h_rgb_buf
+    if h_iter_buf is not None:
+        h_iter_buf[...] = frame.iter_map.cpu().numpy()
+    return out
+
+
+def render_frame_cpu(*args, **kwargs):
+    """The reference's Numba CPU renderer has no product counterpart here: the
+    CPU restatement lives in oracle/ and serves only as checker and baseline."""
+    raise NotImplementedError("render_frame_cpu: fractura_amd has no CPU render path")
+
+
+def recolor_frame(params, d_iter_buf=None, h_iter_buf=None, device=0):
+    """Re-colour a cached float32 map without recomputing iterations
+    (math_core/engine.py:2138-2223): histogram CDF for scheme 11, the palette
+    transfer function otherwise."""
+    import torch
+
+    d_iter = d_iter_buf if d_iter_buf is not None else getattr(params, "d_iter_buf", None)
+    h_iter = h_iter_buf if h_iter_buf is not None else getattr(params, "h_iter_buf", None)
+    dev = get_device(device)
+    with dev.lock:
+        dev.set_stream(params.stream if params.stream is not None else 0)
+        slot = get_stream_slot(params.stream, getattr(params, "stream_slot", None))
+        W, H = int(params.target_w), int(params.target_h)
+        if d_iter is None:
+            if h_iter is None:
+                raise ValueError("recolor_frame needs d_iter_buf or h_iter_buf")
+            d_iter = torch.from_numpy(np.ascontiguousarray(h_iter, dtype=np.float32)).to(dev.torch_device)
+        dt = np.uint16 if params.is_hdr else np.uint8
+        rgb = params.d_rgb_buf if params.d_rgb_buf is not None else get_scratch_device_array((H, W, 3), dt, f"out_rgb_{slot}", device)
+        lut = get_gpu_cmap_lut(params.cmap_name, 2048, bool(params.is_hdr), dev.torch_device)
+        v = _abi.ApView()
+        v.width, v.height, v.max_iter = W, H, int(params.max_iter)
+        v.scheme_id, v.is_hdr = int(params.color_scheme_id), 1 if params.is_hdr else 0
+        v.palette_offset, v.color_density, v.color_contrast = float(params.palette_offset), float(params.color_density), float(params.color_contrast)
+        cdf = None
+        nb = 0
+        if v.scheme_id == HISTOGRAM_SCHEME:
+            hbuf = get_scratch_device_array((HISTOGRAM_BINS,), np.int32, f"hist_bins_{slot}", device)
+            cdf = get_scratch_device_array((HISTOGRAM_BINS,), np.float32, f"cdf_lut_{slot}", device)
+            dev.call("ap_hist_build", ptr(d_iter), W * H, v.max_iter, HISTOGRAM_BINS, ptr(hbuf))
+            dev.call("ap_hist_cdf", ptr(hbuf), HISTOGRAM_BINS, ptr(cdf))
+            nb = HISTOGRAM_BINS
+        dev.call("ap_colorize", C.byref(v), ptr(d_iter), ptr(cdf), nb, ptr(lut), ptr(rgb))
+        out = rgb.cpu().numpy()
+        if params.h_rgb_buf is not None and params.h_rgb_buf.shape == out.shape:
+            params.h_rgb_buf[...] = out
+            out = params.h_rgb_buf
+        return out
+
+
+def compute_dynamic_max_iter(base_max_iter, plot_width_dec, dynamic_mode,
+                             center_x_dec=Decimal("0.0"), center_y_dec=Decimal("0.0"),
+                             **probe_kwargs):
+    """Zoom-dependent max_iter (math_core/engine.py:1284-1323).  The adaptive
+    probe mode is not built here and keeps the base value, as the reference
+    does when its probe fails."""
+    clamped = min(GLOBAL_MAX_ITER, max(1, int(base_max_iter)))
+    if dynamic_mode == "Off" or not dynamic_mode or base_max_iter <= 0:
+        return clamped
+    mode = str(dynamic_mode).lower()
+    if "adaptive" in mode:
+        return clamped
+    try:
+        p = Decimal(str(plot_width_dec))
+        if p <= 0 or p >= Decimal("4.0"):
+            return clamped
+        mant = float(p.scaleb(-p.adjusted()))
+        depth = max(0.0, float(-p.adjusted()) + math.log10(4.0) - math.log10(max(1e-12, mant)))
+        if "sqrt" in mode:
+            mult = 1.0 + 0.85 * math.sqrt(depth)
+        elif "linear" in mode or "log" in mode:
+            mult = 1.0 + 0.35 * depth
+        else:
+            mult = 1.0
+        return min(GLOBAL_MAX_ITER, max(clamped, int(round(clamped * mult))))
+    except (ArithmeticError, ValueError):
+        return clamped

@@ -1,0 +1,251 @@
+"""Host-side arbitrary-precision reference orbits (input to the GPU path).
+
+The reference computes Z_k in MPFR (gmpy2) or Decimal and stores float(Z_k)
+(math_core/high_precision.py:706-875).  Here the orbit runs in binary fixed
+point on Python integers: Z = X / 2^P with P = precision bits + 64 guard
+bits, products truncated by a right shift.  float(Z) is the correctly
+rounded int/int division.  The algorithm (storage until |float Z|^2 > 65536,
+period detection on exact equality at doubling checkpoints for types 0/2,
+Z_ref_len stored after the loop) follows the reference; only the rounding of
+the big-number arithmetic differs, far below the stored double's precision.
+"""
+import threading
+from decimal import Decimal
+
+import numpy as np
+
+from .config import compute_required_prec_bits, decimal_context
+
+GUARD_BITS = 64
+
+
+def to_dd(dec_val):
+    """Decimal -> double-double (hi, lo) (math_core/high_precision.py:17-21)."""
+    d = Decimal(dec_val)
+    hi = float(d)
+    return hi, float(d - Decimal(hi))
+
+
+def _fixed(dec, P):
+    """Round a Decimal (or str/float) to a fixed-point integer with P fraction bits."""
+    d = Decimal(str(dec)) if not isinstance(dec, Decimal) else dec
+    # Exact: scale by 2^P as an integer, then round half-even.
+    sign, digits, exp = d.as_tuple()
+    n = int("".join(map(str, digits))) if digits else 0
+    if exp >= 0:
+        v = (n * 10 ** exp) << P
+    else:
+        num = n << P
+        den = 10 ** (-exp)
+        q, r = divmod(num, den)
+        if 2 * r > den or (2 * r == den and q & 1):
+            q += 1
+        v = q
+    return -v if sign else v
+
+
+def _to_float(X, P):
+    return X / (1 << P)
+
+
+def compute_single_highprec_orbit(cand_cx_dec, cand_cy_dec, max_iter, fractal_type=0,
+                                  julia_cx=0.0, julia_cy=0.0, prec_bits=256,
+                                  gen_n=3, gen_kr=0.25, gen_ki=1.0):
+    """Reference orbit float64 arrays (re, im) of length max_iter+1 and its
+    escape index ref_len (math_core/high_precision.py:706-875)."""
+    P = int(prec_bits) + GUARD_BITS
+    ref_re = np.zeros(max_iter + 1, dtype=np.float64)
+    ref_im = np.zeros(max_iter + 1, dtype=np.float64)
+    if fractal_type in (2, 3):
+        X, Y = _fixed(cand_cx_dec, P), _fixed(cand_cy_dec, P)
+        CX, CY = _fixed(julia_cx, P), _fixed(julia_cy, P)
+    else:
+        X, Y = 0, 0
+        CX, CY = _fixed(cand_cx_dec, P), _fixed(cand_cy_dec, P)
+    one = 1 << P
+    ref_len = max_iter
+    sh = P
+    if fractal_type in (0, 2):
+        check_step = 8
+        saved_k = 0
+        sX, sY = X, Y
+        sfx, sfy = 0.0, 0.0
+        for k in range(max_iter):
+            rx, ry = X / one, Y / one
+            ref_re[k] = rx
+            ref_im[k] = ry
+            if rx * rx + ry * ry > 65536.0:
+                ref_len = k
+                break
+            if k == check_step:
+                check_step <<= 1
+                saved_k = k
+                sX, sY, sfx, sfy = X, Y, rx, ry
+            elif k > saved_k and rx == sfx and ry == sfy and X == sX and Y == sY:
+                period = k - saved_k
+                for j in range(k, max_iter + 1):
+                    ref_re[j] = ref_re[j - period]
+                    ref_im[j] = ref_im[j - period]
+                return ref_re, ref_im, max_iter
+            X2 = (X * X) >> sh
+            Y2 = (Y * Y) >> sh
+            Y = ((X * Y) >> (sh - 1)) + CY
+            X = X2 - Y2 + CX
+        ref_re[ref_len] = X / one
+        ref_im[ref_len] = Y / one
+    elif fractal_type in (1, 3):
+        for k in range(max_iter):
+            rx, ry = X / one, Y / one
+            ref_re[k] = rx
+            ref_im[k] = ry
+            if rx * rx + ry * ry > 65536.0:
+                ref_len = k
+                break
+            X2 = (X * X) >> sh
+            Y2 = (Y * Y) >> sh
+            Y = ((abs(X) * abs(Y)) >> (sh - 1)) - CY
+            X = X2 - Y2 + CX
+        ref_re[ref_len] = X / one
+        ref_im[ref_len] = Y / one
+    else:
+        KR, KI = _fixed(gen_kr, P), _fixed(gen_ki, P)
+        for k in range(max_iter):
+            rx, ry = X / one, Y / one
+            ref_re[k] = rx
+            ref_im[k] = ry
+            if rx * rx + ry * ry > 65536.0:
+                ref_len = k
+                break
+            pr, pi = X, Y
+            for _ in range(gen_n - 1):
+                pr, pi = (pr * X - pi * Y) >> sh, (pr * Y + pi * X) >> sh
+            kzr = (KR * X - KI * Y) >> sh
+            kzi = (KR * Y + KI * X) >> sh
+            X, Y = pr + kzr + CX, pi + kzi + CY
+        ref_re[ref_len] = X / one
+        ref_im[ref_len] = Y / one
+    return ref_re, ref_im, ref_len
+
+
+def compute_highprec_escape_scalar(cx_dec, cy_dec, max_iter, fractal_type=0, julia_cx=0.0,
+                                   julia_cy=0.0, prec_bits=256, gen_n=3, gen_kr=0.25, gen_ki=1.0):
+    """Escape index of one point at high precision (max_iter if it stays
+    bounded), the probe of math_core/high_precision.py:581-703."""
+    _, _, n = compute_single_highprec_orbit(cx_dec, cy_dec, max_iter, fractal_type, julia_cx,
+                                            julia_cy, prec_bits, gen_n, gen_kr, gen_ki)
+    return float(n)
+
+
+_CACHE = []
+_CACHE_LOCK = threading.Lock()
+_CACHE_SIZE = 16
+
+_PROBE_UV = [
+    (0.35, 0.35), (0.65, 0.35), (0.35, 0.65), (0.65, 0.65),
+    (0.5, 0.25), (0.5, 0.75), (0.25, 0.5), (0.75, 0.5),
+    (0.2, 0.2), (0.8, 0.2), (0.2, 0.8), (0.8, 0.8),
+    (0.5, 0.35), (0.5, 0.65), (0.35, 0.5), (0.65, 0.5),
+]
+
+
+def clear_reference_orbit_cache():
+    with _CACHE_LOCK:
+        _CACHE.clear()
+
+
+def cache_reference_orbit(cx_dec, cy_dec, pw_dec, max_iter, ref_re, ref_im, ref_len,
+                          fractal_type=0, julia_cx=0.0, julia_cy=0.0, gen_n=3, gen_kr=0.25,
+                          gen_ki=1.0, is_floatexp=False):
+    """Insert an orbit at the front of the 16-entry LRU list
+    (math_core/high_precision.py:893-916)."""
+    with _CACHE_LOCK:
+        _CACHE.insert(0, dict(is_floatexp=is_floatexp, fractal_type=fractal_type,
+                              julia_cx=julia_cx, julia_cy=julia_cy, gen_n=gen_n, gen_kr=gen_kr,
+                              gen_ki=gen_ki, pw_dec=pw_dec, cx_dec=cx_dec, cy_dec=cy_dec,
+                              ref_len=ref_len, max_iter=max_iter, ref_re=ref_re, ref_im=ref_im))
+        if len(_CACHE) > _CACHE_SIZE:
+            _CACHE.pop()
+
+
+def find_optimal_reference_orbit(cx_dec, cy_dec, pw_dec, ph_dec, max_iter, screen_w, screen_h,
+                                 fractal_type=0, julia_cx=0.0, julia_cy=0.0, gen_n=3,
+                                 gen_kr=0.25, gen_ki=1.0, is_floatexp=False, probe=True):
+    """Reference point and orbit for a view: cache reuse, then the centre, then
+    (if the centre escapes before 60% of max_iter) the best of 16 probes
+    (math_core/high_precision.py:919-1097).  Returns
+    (cx_dec, cy_dec, u, v, ref_re, ref_im, ref_len) with (u, v) the reference's
+    position as a fraction of the view."""
+    with decimal_context(pw_dec):
+        x_min = cx_dec - pw_dec / Decimal(2)
+        y_max = cy_dec + ph_dec / Decimal(2)
+        best_entry, best_score = None, -1
+        with _CACHE_LOCK:
+            for e in _CACHE:
+                if e["is_floatexp"] != is_floatexp:
+                    continue
+                if (e["fractal_type"], e["julia_cx"], e["julia_cy"], e["gen_n"], e["gen_kr"],
+                        e["gen_ki"]) != (fractal_type, julia_cx, julia_cy, gen_n, gen_kr, gen_ki):
+                    continue
+                ratio = pw_dec / e["pw_dec"]
+                if not (Decimal("0.25") <= ratio <= Decimal("4.0")):
+                    continue
+                cu = float((e["cx_dec"] - x_min) / pw_dec)
+                cv = float((y_max - e["cy_dec"]) / ph_dec)
+                if 0.10 <= cu <= 0.90 and 0.10 <= cv <= 0.90 and e["ref_len"] > best_score:
+                    best_score, best_entry = e["ref_len"], e
+                    if e["ref_len"] >= max_iter:
+                        break
+        can_reuse = False
+        if best_entry is not None:
+            natural = best_entry["ref_len"] < best_entry.get("max_iter", best_entry["ref_len"])
+            ratio = pw_dec / best_entry["pw_dec"]
+            panning = Decimal("0.75") <= ratio <= Decimal("1.33")
+            full_or_natural = best_entry["ref_len"] >= max_iter or natural
+            if best_score >= max_iter:
+                can_reuse = True
+            elif panning and best_score >= 100 and full_or_natural:
+                can_reuse = True
+            elif natural and best_score >= int(max_iter * 0.35):
+                can_reuse = True
+            elif (best_entry["cx_dec"], best_entry["cy_dec"], best_entry["pw_dec"]) == (cx_dec, cy_dec, pw_dec) \
+                    and full_or_natural:
+                can_reuse = True
+
+        def reuse():
+            e = best_entry
+            u = float((e["cx_dec"] - x_min) / pw_dec)
+            v = float((y_max - e["cy_dec"]) / ph_dec)
+            return e["cx_dec"], e["cy_dec"], u, v, e["ref_re"], e["ref_im"], min(e["ref_len"], max_iter)
+
+        if can_reuse:
+            return reuse()
+
+        bits = compute_required_prec_bits(pw_dec)
+        ref_re, ref_im, ref_len = compute_single_highprec_orbit(
+            cx_dec, cy_dec, max_iter, fractal_type, julia_cx, julia_cy, bits, gen_n, gen_kr, gen_ki)
+        best = (cx_dec, cy_dec, 0.5, 0.5)
+        if probe and ref_len < int(max_iter * 0.6) and pw_dec >= Decimal("1e-10000"):
+            results = []
+            for u, v in _PROBE_UV:
+                px = x_min + Decimal(str(u)) * pw_dec
+                py = y_max - Decimal(str(v)) * ph_dec
+                esc = compute_highprec_escape_scalar(px, py, max_iter, fractal_type, julia_cx,
+                                                     julia_cy, bits, gen_n, gen_kr, gen_ki)
+                results.append((esc, px, py, u, v))
+            best_esc = max(int(ref_len * 1.2), ref_len + 32)
+            cand = None
+            for esc, px, py, u, v in results:
+                if esc > best_esc:
+                    best_esc, cand = esc, (px, py, u, v)
+            if cand is not None:
+                c_re, c_im, c_len = compute_single_highprec_orbit(
+                    cand[0], cand[1], max_iter, fractal_type, julia_cx, julia_cy, bits, gen_n,
+                    gen_kr, gen_ki)
+                if c_len > ref_len:
+                    ref_re, ref_im, ref_len = c_re, c_im, c_len
+                    best = cand
+        if can_reuse and best_entry is not None and best_entry["ref_len"] > ref_len:
+            return reuse()
+        cache_reference_orbit(best[0], best[1], pw_dec, max_iter, ref_re, ref_im, ref_len,
+                              fractal_type, julia_cx, julia_cy, gen_n, gen_kr, gen_ki, is_floatexp)
+        return best[0], best[1], best[2], best[3], ref_re, ref_im, ref_len

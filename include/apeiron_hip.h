@@ -1,0 +1,219 @@
+/*
+ * apeiron_hip.h -- C ABI of the MI355X (gfx950) escape-time hot path.
+ *
+ * One shared library, libapeiron_hip.so, built with hipcc for gfx950.  Every
+ * entry point takes plain pointers and sizes; device pointers are owned by the
+ * caller (the Python host allocates them through torch, a C caller through
+ * ap_alloc).  Every call returns an int status (AP_OK == 0, negative on
+ * error); the text of the last error of a context is ap_last_error().  No C++
+ * exception crosses this boundary.  All launches are asynchronous on the
+ * context's stream; ap_sync() waits for them.
+ *
+ * A context is not shared between threads without external locking;
+ * separate contexts on separate streams are re-entrant.
+ *
+ * Each entry point names the reference interface it replaces.  The reference
+ * is a Numba program: its kernels are launched as
+ * `kernel[blocks, threads, stream](...)` from math_core/engine.py, and its
+ * host-side table builder is a Numba @njit function.
+ */
+#ifndef APEIRON_HIP_H
+#define APEIRON_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AP_OK 0
+#define AP_ERR_ARG (-1)         /* bad argument (null pointer, size, enum) */
+#define AP_ERR_HIP (-2)         /* a HIP runtime call failed               */
+#define AP_ERR_UNSUPPORTED (-3) /* combination not implemented             */
+
+/* Fractal types (reference config.py:78-84). */
+#define AP_MANDELBROT 0
+#define AP_BURNING_SHIP 1
+#define AP_JULIA_MANDELBROT 2
+#define AP_JULIA_BURNING_SHIP 3
+#define AP_GENERAL_MANDELBROT 4
+
+/* Pixel-pass modes (reference math_core/cuda_kernels.py:1615-1664,
+ * 1736-1824): 0 = one centred sample per pixel; 1 = factor^2 un-centred
+ * subsamples for every selected pixel (rebase passes with factor > 1);
+ * 2 = factor^2 subsamples only where mask != 0 (adaptive-SSAA pass 3). */
+#define AP_MODE_CENTRE 0
+#define AP_MODE_ALL_SUBSAMPLES 1
+#define AP_MODE_EDGE_SUBSAMPLES 2
+
+/* Per-launch statistics slots (uint64 accumulators in ap_outputs.stats). */
+#define AP_STAT_ITERATIONS 0 /* sum over samples of iterations advanced   */
+#define AP_STAT_PLAIN_STEPS 1 /* executed single-iteration steps          */
+#define AP_STAT_BLA_STEPS 2  /* executed BLA leaps                         */
+#define AP_STAT_SAMPLES 3    /* evaluated samples                          */
+#define AP_STAT_COUNT 4
+
+typedef struct ap_ctx ap_ctx;
+
+/*
+ * Everything a pixel pass needs to know about the view.  The grid constants
+ * are the doubles the reference host derives from Decimal centre/width
+ * (math_core/engine.py:1386-1421, 1643-1648; math_core/cpu_kernels.py:1120):
+ * sample (sub_i, sub_j) sits at x = x_min + sub_i*dx, y = y_max - sub_j*dy,
+ * where sub_i = i*factor + (factor-1)/2 in mode 0 and i*factor + sx in modes
+ * 1 and 2.  The *_lo fields are the low halves of the double-double grid
+ * (used by ap_render_dd only).
+ */
+typedef struct ap_view {
+    int32_t width;        /* target_w                                        */
+    int32_t height;       /* target_h                                        */
+    int32_t factor;       /* SSAA factor >= 1                                */
+    int32_t max_iter;
+    int32_t fractal_type; /* AP_MANDELBROT ... AP_GENERAL_MANDELBROT          */
+    int32_t gen_n;        /* exponent n of z^n + k z + c (type 4)            */
+    int32_t mode;         /* AP_MODE_*                                       */
+    int32_t rebase_pass;  /* > 0: only pixels whose ap_outputs.mask != 0     */
+    double gen_kr, gen_ki;
+    double julia_cx, julia_cy;       /* Julia parameter (hi parts for DD)   */
+    double julia_cx_lo, julia_cy_lo;
+    double x_min, y_max;             /* grid origin (hi parts for DD)       */
+    double x_min_lo, y_max_lo;
+    double dx, dy;                   /* subsample spacing (hi parts for DD) */
+    double dx_lo, dy_lo;
+    double inv_ln2;         /* 1/ln 2 as the reference passes it (f64)      */
+    double ln_bailout_log;  /* ln(0.5 ln 65536) as the reference passes it   */
+    /* perturbation engines */
+    double ref_px, ref_py;  /* reference position in subsample units        */
+    double ref_cx, ref_cy;  /* float(reference centre)                       */
+    int32_t ref_len;        /* escape index of the reference orbit          */
+    int32_t e_scale;        /* FloatExp scale E: dc is passed times 2^E      */
+    int32_t strict_glitch;  /* Pauldelbrot test on (1) or off (0)           */
+    int32_t bla_levels;     /* levels in the BLA table; 0 = no BLA          */
+    /* colouring */
+    int32_t scheme_id;      /* palette transfer function 0..11              */
+    int32_t is_hdr;         /* rgb is uint16 (LUT x65535) instead of uint8  */
+    double palette_offset, color_density, color_contrast;
+} ap_view;
+
+/*
+ * Output (and side-input) buffers of a pixel pass; every pointer is a device
+ * pointer and may be NULL when not wanted.  Layouts are row-major (H, W).
+ */
+typedef struct ap_outputs {
+    int32_t *counts;      /* escape index k; max_iter if never escaped;
+                             -1 where the cardioid/bulb shortcut applied     */
+    double *smooth;       /* float64 smooth value k+1-log2(ln|z|/ln 256);
+                             -1 for the interior                             */
+    float *iter_map;      /* reference float32 smooth map (-1 interior)      */
+    uint8_t *glitch;      /* glitch flags (perturbation engines)             */
+    void *rgb;            /* (H, W, 3) uint8, or uint16 if view.is_hdr       */
+    const float *lut;     /* (2048, 3) float32 colour LUT                    */
+    const uint8_t *mask;  /* edge mask (mode 2) or glitch-in mask (rebase)   */
+    const float *cdf;     /* histogram CDF: colour subsamples by scheme 11   */
+    int32_t num_bins;     /* CDF length when cdf != NULL                     */
+    int32_t reserved0;
+    const int32_t *queue;       /* optional compacted pixel list (y*W + x)  */
+    const int32_t *queue_count; /* device count of entries in queue         */
+    unsigned long long *stats;  /* optional AP_STAT_COUNT accumulators      */
+} ap_outputs;
+
+/* Reference orbit Z_0..Z_ref_len in float64 (re and im arrays of length
+ * ref_len+1), as math_core/high_precision.py:706-875 produces it. */
+typedef struct ap_orbit {
+    const double *re;
+    const double *im;
+    int32_t len;          /* ref_len; arrays hold len+1 entries              */
+    int32_t reserved0;
+} ap_orbit;
+
+/* ---- context, stream and memory ------------------------------------------
+ * Replaces init_primary_cuda_device (math_core/cuda_kernels.py:11-30), the
+ * Numba stream/device-array plumbing and the scratch pool
+ * (math_core/engine.py:94-188). */
+int ap_init(int device, ap_ctx **out);
+int ap_destroy(ap_ctx *ctx);
+const char *ap_last_error(const ap_ctx *ctx);
+int ap_set_stream(ap_ctx *ctx, void *hip_stream); /* NULL = default stream */
+int ap_sync(ap_ctx *ctx);
+int ap_alloc(ap_ctx *ctx, size_t bytes, void **dptr);
+int ap_free(ap_ctx *ctx, void *dptr);
+int ap_memcpy_h2d(ap_ctx *ctx, void *dst, const void *src, size_t bytes);
+int ap_memcpy_d2h(ap_ctx *ctx, void *dst, const void *src, size_t bytes);
+int ap_memset(ap_ctx *ctx, void *dst, int value, size_t bytes);
+int ap_device_info(ap_ctx *ctx, int *num_cus, int *clock_khz, char *name, size_t name_len);
+
+/* ---- direct engines --------------------------------------------------------
+ * FP64: replaces mandelbrot_unified_fp64_kernel (math_core/cuda_kernels.py:
+ * 1615-1664) over compute_fp64_sample (:263-317); restates
+ * cpu_compute_fp64_sample (math_core/cpu_kernels.py:227-266) bit-exactly.
+ * DD: replaces mandelbrot_unified_dd_kernel (math_core/cuda_kernels.py:
+ * 1666-1734) over compute_dd_sample (:320-418). */
+int ap_render_fp64(ap_ctx *ctx, const ap_view *view, const ap_outputs *out);
+int ap_render_dd(ap_ctx *ctx, const ap_view *view, const ap_outputs *out);
+
+/* ---- perturbation engines --------------------------------------------------
+ * PERT64: FP64-delta perturbation, replaces mandelbrot_unified_perturbation_
+ * kernel (math_core/cuda_kernels.py:1736-1824) over compute_perturbation_
+ * sample_bla (:593-815); restates cpu_compute_perturbation_sample
+ * (math_core/cpu_kernels.py:373-602).
+ * FEXP64: FloatExp with an FP64 mantissa and int32 exponent, replaces
+ * mandelbrot_unified_floatexp_kernel (math_core/cuda_kernels.py:1827-1917)
+ * over compute_perturbation_sample_floatexp_bla (:1101-1499); restates
+ * cpu_compute_perturbation_sample_floatexp (math_core/cpu_kernels.py:605-1021).
+ * FEXP32: the same loop with an FP32 mantissa (no reference counterpart; the
+ * reference FloatExp mantissa is FP64).
+ * `bla` is a table from ap_bla_build (NULL or view.bla_levels == 0: none). */
+int ap_render_pert64(ap_ctx *ctx, const ap_view *view, const ap_orbit *orbit,
+                     const void *bla, const ap_outputs *out);
+int ap_render_fexp64(ap_ctx *ctx, const ap_view *view, const ap_orbit *orbit,
+                     const void *bla, const ap_outputs *out);
+int ap_render_fexp32(ap_ctx *ctx, const ap_view *view, const ap_orbit *orbit,
+                     const void *bla, const ap_outputs *out);
+
+/* ---- chained BLA table -----------------------------------------------------
+ * Replaces _jit_build_bla_table (math_core/high_precision.py:1100-1238) and
+ * upload_bla_table_cuda (math_core/engine.py:416-423): the table is built on
+ * the device, one launch per dyadic level.  ap_bla_bytes gives the size of
+ * the caller-allocated device buffer; the layout is this library's own.
+ * ap_bla_export writes the reference layout float64[levels][ref_len][5]
+ * (A_re, A_im, B_re, B_im, r^2) into a device buffer, for comparison. */
+size_t ap_bla_bytes(int32_t ref_len, int32_t levels);
+int ap_bla_build(ap_ctx *ctx, const ap_orbit *orbit, int32_t fractal_type,
+                 int32_t gen_n, double gen_kr, double gen_ki, double tol,
+                 int32_t levels, void *table);
+int ap_bla_export(ap_ctx *ctx, const void *table, int32_t ref_len,
+                  int32_t levels, double *out_ref_layout);
+
+/* ---- adaptive SSAA, glitch queues ------------------------------------------
+ * ap_edge_mask replaces edge_detect_kernel (math_core/cuda_kernels.py:
+ * 1587-1611).  ap_compact replaces the host np.argwhere over a copied-back
+ * mask (math_core/engine.py:1902-1905): indices (y*W + x) of non-zero mask
+ * entries, in ascending order, and their count, stay on the device.
+ * ap_glitch_bins counts mask entries in the 16x16 spatial bins of the rebase
+ * point selection (math_core/engine.py:1924-1928). */
+int ap_edge_mask(ap_ctx *ctx, const float *iter_map, int32_t width,
+                 int32_t height, float threshold, uint8_t *mask);
+int ap_compact(ap_ctx *ctx, const uint8_t *mask, int32_t n, int32_t *queue,
+               int32_t *count);
+int ap_glitch_bins(ap_ctx *ctx, const uint8_t *mask, int32_t width,
+                   int32_t height, int32_t *bins256);
+
+/* ---- histogram colouring ---------------------------------------------------
+ * ap_hist_build replaces zero_histogram_cuda_kernel + build_histogram_kernel
+ * (math_core/cuda_kernels.py:1922-1926, 1999-2013); ap_hist_cdf replaces the
+ * scan_blocks / scan_single_block / finalize_normalized_cdf kernels
+ * (:1929-1996); ap_colorize replaces apply_histogram_equalization_kernel
+ * (:2016-2035) for scheme 11 and recolor_iter_map_cuda_kernel (:3201-3222)
+ * for schemes 0..10. */
+int ap_hist_build(ap_ctx *ctx, const float *iter_map, int32_t n,
+                  int32_t max_iter, int32_t num_bins, int32_t *hist);
+int ap_hist_cdf(ap_ctx *ctx, const int32_t *hist, int32_t num_bins, float *cdf);
+int ap_colorize(ap_ctx *ctx, const ap_view *view, const float *iter_map,
+                const float *cdf, int32_t num_bins, const float *lut, void *rgb);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* APEIRON_HIP_H */

@@ -1,0 +1,941 @@
+// CPU oracle of the escape-time hot path -- TEST INFRASTRUCTURE ONLY.
+//
+// A C++ restatement of the reference's CPU (Numba) path, compiled with
+// -ffp-contract=off -fno-fast-math so every multiply and add rounds once, as
+// written.  It is imported only by tests/, by __graft_entry__.smoke() and by
+// bench.py's cpu_baseline leg, and only as the checker or the CPU baseline;
+// the product path (fractura_amd) never loads it.
+//
+// Each function names the reference lines it restates.  Parity status:
+//   fp64 sample / render ......... pinned: closed forms (tests/test_oracle.py)
+//   dd sample .................... pinned: mpmath 200-digit direct loop
+//   perturbation FP64 ........... pinned: equals the FP64 direct loop when the
+//                                   reference point is the pixel; mpmath
+//   FloatExp FP64 ................ pinned: equals the FP64-delta path at E = 0;
+//                                   mpmath perturbation oracle
+//   BLA table .................... pinned: brute-force composition in exact
+//                                   rationals on short orbits
+//   histogram / cdf / colour ..... pinned: numpy restatements + hand cases
+//   edge mask .................... pinned: numpy brute force
+// The Pauldelbrot exponent uses int(log(x) * 1/ln 2) as the reference does.
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "apeiron_hip.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+const double kBail = 65536.0;
+
+struct Res {
+    int count;
+    float s;
+    double smooth;
+    int glitch;
+    long long iters;
+};
+
+Res interior_res(int count, long long iters) { return Res{count, -1.0f, -1.0, 0, iters}; }
+
+// Escape epilogue, math_core/cpu_kernels.py:246-249.
+Res escape_res(int k, double mag2, double inv_ref, double inv_exact, double ln_bail) {
+    Res r;
+    r.count = k;
+    r.glitch = 0;
+    r.iters = k;
+    double lm = log(mag2);
+    if (lm > 1.0) {
+        double nu = log(0.5 * lm) - ln_bail;
+        r.s = (float)(k + 1.0) - (float)(nu * inv_ref);
+        r.smooth = (double)(k + 1) - nu * inv_exact;
+    } else {
+        r.s = (float)k;
+        r.smooth = (double)k;
+    }
+    return r;
+}
+
+Res nan_res(int k) { return Res{k, (float)k, (double)k, 1, k}; }
+
+void inv_ln(const ap_view& v, double& ref, double& exact) {
+    ref = exact = v.inv_ln2;
+    if (v.fractal_type == 4 && v.gen_n > 1) {
+        exact = 1.0 / log((double)v.gen_n);
+        ref = (double)(float)exact;
+    }
+}
+
+// ------------------------------------------------ FP64 direct sample --------
+// cpu_compute_fp64_sample, math_core/cpu_kernels.py:227-266.
+Res fp64_sample(const ap_view& v, double x, double y) {
+    const int type = v.fractal_type;
+    if (type == 0) {
+        double xm = x - 0.25, y2 = y * y;
+        double q = xm * xm + y2;
+        if ((q * (q + xm) <= 0.25 * y2) || ((x + 1.0) * (x + 1.0) + y2 <= 0.0625))
+            return interior_res(-1, 0);
+    }
+    double zx, zy, cx, cy;
+    if (type == 2 || type == 3) {
+        zx = x; zy = y; cx = v.julia_cx; cy = v.julia_cy;
+    } else {
+        zx = 0.0; zy = 0.0; cx = x; cy = y;
+    }
+    double ir, ie;
+    inv_ln(v, ir, ie);
+    for (int k = 0; k < v.max_iter; ++k) {
+        double zx2 = zx * zx, zy2 = zy * zy;
+        double mag2 = zx2 + zy2;
+        if (mag2 > kBail) return escape_res(k, mag2, ir, ie, v.ln_bailout_log);
+        if (type == 0 || type == 2) {
+            zy = 2.0 * zx * zy + cy;
+            zx = zx2 - zy2 + cx;
+        } else if (type == 1 || type == 3) {
+            zy = 2.0 * fabs(zx) * fabs(zy) - cy;
+            zx = zx2 - zy2 + cx;
+        } else {
+            double pr = zx, pi = zy;
+            for (int e = 0; e < v.gen_n - 1; ++e) {
+                double t = pr * zx - pi * zy;
+                pi = pr * zy + pi * zx;
+                pr = t;
+            }
+            double kzr = v.gen_kr * zx - v.gen_ki * zy;
+            double kzi = v.gen_kr * zy + v.gen_ki * zx;
+            zx = pr + kzr + cx;
+            zy = pi + kzi + cy;
+        }
+    }
+    return interior_res(v.max_iter, v.max_iter);
+}
+
+// ------------------------------------------------ double-double -------------
+// math_core/cpu_kernels.py:95-158 (Dekker split two_prod).
+struct DD {
+    double hi, lo;
+};
+DD two_sum(double a, double b) {
+    double s = a + b, v = s - a;
+    return {s, (a - (s - v)) + (b - v)};
+}
+DD two_prod(double a, double b) {
+    double c = 134217729.0 * a;
+    double ah = c - (c - a), al = a - ah;
+    c = 134217729.0 * b;
+    double bh = c - (c - b), bl = b - bh;
+    double p = a * b;
+    double e = ((ah * bh - p) + ah * bl + al * bh) + al * bl;
+    return {p, e};
+}
+DD dd_add(DD a, DD b) {
+    DD t = two_sum(a.hi, b.hi);
+    return two_sum(t.hi, t.lo + a.lo + b.lo);
+}
+DD dd_sub(DD a, DD b) {
+    DD t = two_sum(a.hi, -b.hi);
+    return two_sum(t.hi, t.lo + a.lo - b.lo);
+}
+DD dd_mul(DD a, DD b) {
+    DD p = two_prod(a.hi, b.hi);
+    return two_sum(p.hi, p.lo + a.hi * b.lo + a.lo * b.hi);
+}
+DD dd_mul_f64(DD a, double b) {
+    DD p = two_prod(a.hi, b);
+    return two_sum(p.hi, p.lo + a.lo * b);
+}
+DD dd_sqr(DD a) {
+    DD p = two_prod(a.hi, a.hi);
+    return two_sum(p.hi, p.lo + 2.0 * a.hi * a.lo);
+}
+DD dd_abs(DD a) { return (a.hi < 0.0 || (a.hi == 0.0 && a.lo < 0.0)) ? DD{-a.hi, -a.lo} : a; }
+
+// cpu_compute_dd_sample, math_core/cpu_kernels.py:270-369.
+Res dd_sample(const ap_view& v, DD cxi, DD cyi) {
+    const int type = v.fractal_type;
+    if (type == 0) {
+        double xm = cxi.hi - 0.25, y2 = cyi.hi * cyi.hi;
+        double q = xm * xm + y2;
+        if ((q * (q + xm) <= 0.25 * y2) || ((cxi.hi + 1.0) * (cxi.hi + 1.0) + y2 <= 0.0625))
+            return interior_res(-1, 0);
+    }
+    DD zx, zy, cx, cy;
+    if (type == 2 || type == 3) {
+        zx = cxi; zy = cyi;
+        cx = DD{v.julia_cx, v.julia_cx_lo};
+        cy = DD{v.julia_cy, v.julia_cy_lo};
+    } else {
+        zx = DD{0.0, 0.0}; zy = DD{0.0, 0.0};
+        cx = cxi; cy = cyi;
+    }
+    double ir, ie;
+    inv_ln(v, ir, ie);
+    for (int k = 0; k < v.max_iter; ++k) {
+        DD zx2 = dd_sqr(zx), zy2 = dd_sqr(zy);
+        double mag2 = zx2.hi + zy2.hi;
+        if (mag2 > kBail) return escape_res(k, mag2, ir, ie, v.ln_bailout_log);
+        if (type == 0 || type == 2) {
+            DD sub = dd_sub(zx2, zy2);
+            DD zxy = dd_mul(zx, zy);
+            zx = dd_add(sub, cx);
+            zy = dd_add(DD{zxy.hi * 2.0, zxy.lo * 2.0}, cy);
+        } else if (type == 1 || type == 3) {
+            DD sub = dd_sub(zx2, zy2);
+            DD zxy = dd_mul(dd_abs(zx), dd_abs(zy));
+            zx = dd_add(sub, cx);
+            zy = dd_sub(DD{zxy.hi * 2.0, zxy.lo * 2.0}, cy);
+        } else {
+            DD pr = zx, pi = zy;
+            for (int e = 0; e < v.gen_n - 1; ++e) {
+                DD re = dd_sub(dd_mul(pr, zx), dd_mul(pi, zy));
+                DD im = dd_add(dd_mul(pr, zy), dd_mul(pi, zx));
+                pr = re;
+                pi = im;
+            }
+            DD kzr = dd_sub(dd_mul_f64(zx, v.gen_kr), dd_mul_f64(zy, v.gen_ki));
+            DD kzi = dd_add(dd_mul_f64(zy, v.gen_kr), dd_mul_f64(zx, v.gen_ki));
+            zx = dd_add(dd_add(pr, kzr), cx);
+            zy = dd_add(dd_add(pi, kzi), cy);
+        }
+    }
+    return interior_res(v.max_iter, v.max_iter);
+}
+
+// ------------------------------------------------ perturbation --------------
+struct Orbit {
+    const double* zr;
+    const double* zi;
+    int len;
+    const double* bla;  // reference layout [levels][len][5], or null
+    int levels;
+};
+
+inline double B(const Orbit& o, int d, int k, int f) {
+    return o.bla[((size_t)d * o.len + k) * 5 + f];
+}
+
+// Stage 2 (direct continuation), math_core/cpu_kernels.py:541-602.
+Res stage2(const ap_view& v, double zx, double zy, double cx, double cy, int k, int glitch) {
+    double ir, ie;
+    inv_ln(v, ir, ie);
+    const int type = v.fractal_type;
+    while (k < v.max_iter) {
+        double zx2 = zx * zx, zy2 = zy * zy;
+        double mag2 = zx2 + zy2;
+        if (std::isnan(mag2) || std::isinf(mag2)) return nan_res(k);
+        if (mag2 > kBail) {
+            Res r = escape_res(k, mag2, ir, ie, v.ln_bailout_log);
+            r.glitch = glitch;
+            return r;
+        }
+        if (type == 0 || type == 2) {
+            zy = 2.0 * zx * zy + cy;
+            zx = zx2 - zy2 + cx;
+        } else if (type == 1 || type == 3) {
+            zy = 2.0 * fabs(zx) * fabs(zy) - cy;
+            zx = zx2 - zy2 + cx;
+        } else {
+            double pr = zx, pi = zy;
+            for (int e = 0; e < v.gen_n - 1; ++e) {
+                double t = pr * zx - pi * zy;
+                pi = pr * zy + pi * zx;
+                pr = t;
+            }
+            double kzr = v.gen_kr * zx - v.gen_ki * zy;
+            double kzi = v.gen_kr * zy + v.gen_ki * zx;
+            zx = pr + kzr + cx;
+            zy = pi + kzi + cy;
+        }
+        ++k;
+    }
+    Res r = interior_res(v.max_iter, v.max_iter);
+    r.glitch = glitch;
+    return r;
+}
+
+// cpu_compute_perturbation_sample with the BLA table, bbsa_skip = 0
+// (math_core/cpu_kernels.py:373-602).
+Res pert64_sample(const ap_view& v, const Orbit& o, double si, double sj) {
+    const int type = v.fractal_type;
+    const bool julia = type == 2 || type == 3;
+    const double dcy = (v.ref_py - sj) * v.dy;
+    const double dcx = (si - v.ref_px) * v.dx;
+    double dzx = julia ? dcx : 0.0, dzy = julia ? dcy : 0.0;
+    const double dpx = julia ? 0.0 : dcx, dpy = julia ? 0.0 : dcy;
+    const bool strict = v.strict_glitch > 0;
+    const bool has_bla = o.bla && o.levels > 1 && (type == 0 || type == 2 || type == 4);
+    const int ref_len = o.len;
+    double ir, ie;
+    inv_ln(v, ir, ie);
+    int k = 0, glitch = 0;
+    while (k < ref_len) {
+        double rx = o.zr[k], ry = o.zi[k];
+        double zx = rx + dzx, zy = ry + dzy;
+        double mag2 = zx * zx + zy * zy;
+        if (mag2 > kBail) {
+            if (strict && k > 15 && (rx * rx + ry * ry) < 4.0) glitch = 1;
+            Res r = escape_res(k, mag2, ir, ie, v.ln_bailout_log);
+            r.glitch = glitch;
+            return r;
+        }
+        if (strict && k > 15) {
+            double rm2 = rx * rx + ry * ry;
+            if (mag2 < 1e-6 * rm2 && rm2 > 1e-8) glitch = 1;
+        }
+        if (has_bla && k < ref_len - 1) {
+            double dzm2 = dzx * dzx + dzy * dzy;
+            if (dzm2 < B(o, 0, k, 4)) {
+                int best = 0, max_d = o.levels - 1;
+                if (dzm2 == 0.0 && (dzx != 0.0 || dzy != 0.0)) max_d = std::min(8, max_d);
+                for (int d = max_d; d > 0; --d) {
+                    if (k + (1 << d) < ref_len && dzm2 < B(o, d, k, 4)) {
+                        best = d;
+                        break;
+                    }
+                }
+                double ar = B(o, best, k, 0), ai = B(o, best, k, 1);
+                double br = B(o, best, k, 2), bi = B(o, best, k, 3);
+                double ndx = (ar * dzx - ai * dzy) + (br * dpx - bi * dpy);
+                dzy = (ar * dzy + ai * dzx) + (br * dpy + bi * dpx);
+                dzx = ndx;
+                if (std::isnan(dzx) || std::isnan(dzy) || std::isinf(dzx) || std::isinf(dzy))
+                    return nan_res(k);
+                k += 1 << best;
+                continue;
+            }
+        }
+        if (type == 0 || type == 2) {
+            double ndx = (2.0 * rx + dzx) * dzx - (2.0 * ry + dzy) * dzy + dpx;
+            dzy = 2.0 * ((rx + dzx) * dzy + ry * dzx) + dpy;
+            dzx = ndx;
+        } else if (type == 1 || type == 3) {
+            double ndx = (2.0 * rx + dzx) * dzx - (2.0 * ry + dzy) * dzy + dpx;
+            double sxs = rx >= 0.0 ? 1.0 : -1.0, sys = ry >= 0.0 ? 1.0 : -1.0;
+            double ddx = (rx + dzx) * sxs >= 0.0 ? sxs * dzx : -sxs * (2.0 * rx + dzx);
+            double ddy = (ry + dzy) * sys >= 0.0 ? sys * dzy : -sys * (2.0 * ry + dzy);
+            dzy = 2.0 * (fabs(rx) * ddy + fabs(ry) * ddx + ddx * ddy) - dpy;
+            dzx = ndx;
+        } else {
+            double pr = dzx, pi = dzy, qr = rx, qi = ry;
+            for (int e = 0; e < v.gen_n - 1; ++e) {
+                double t1r = pr * rx - pi * ry, t1i = pr * ry + pi * rx;
+                double sr = pr + qr, sim = pi + qi;
+                double t2r = sr * dzx - sim * dzy, t2i = sr * dzy + sim * dzx;
+                pr = t1r + t2r;
+                pi = t1i + t2i;
+                double nq = qr * rx - qi * ry;
+                qi = qr * ry + qi * rx;
+                qr = nq;
+            }
+            double kdr = v.gen_kr * dzx - v.gen_ki * dzy, kdi = v.gen_kr * dzy + v.gen_ki * dzx;
+            dzx = pr + kdr + dpx;
+            dzy = pi + kdi + dpy;
+        }
+        ++k;
+    }
+    double zx = o.zr[ref_len] + dzx, zy = o.zi[ref_len] + dzy;
+    double cx = julia ? v.julia_cx : v.ref_cx + dcx;
+    double cy = julia ? v.julia_cy : v.ref_cy + dcy;
+    if (ref_len < v.max_iter) glitch = 1;
+    return stage2(v, zx, zy, cx, cy, k, glitch);
+}
+
+// pow2_f64, math_core/cpu_kernels.py:161-167.
+double pow2_f64(int e) {
+    if (e >= 1023) return 1.7976931348623157e+308;
+    if (e <= -1022) return 0.0;
+    return ldexp(1.0, e);
+}
+double pymax(double a, double b) { return b > a ? b : a; }
+
+// cpu_check_glitch_cancellation_floatexp, math_core/cpu_kernels.py:170-201.
+int glitch_fexp(double rx, double ry, double dzx, double dzy, int eff) {
+    double rmax = pymax(fabs(rx), fabs(ry));
+    if (rmax < 1e-4) return 0;
+    double dmax = pymax(fabs(dzx), fabs(dzy));
+    if (dmax <= 0.0) return 0;
+    int r_exp = (int)(log(rmax) * 1.4426950408889634);
+    int d_exp = (int)(log(dmax) * 1.4426950408889634);
+    int diff = r_exp - (eff + d_exp);
+    if (diff > 10 || diff < -10) return 0;
+    double sr = pow2_f64(-r_exp), sd = pow2_f64(eff - r_exp);
+    double rxs = rx * sr, rys = ry * sr;
+    double zx = rxs + dzx * sd, zy = rys + dzy * sd;
+    return (zx * zx + zy * zy) < 1e-6 * (rxs * rxs + rys * rys) ? 1 : 0;
+}
+
+void renorm(double& dzx, double& dzy, int& ze, bool guard) {
+    const double inv16 = 1.52587890625e-05;
+    double mag = pymax(fabs(dzx), fabs(dzy));
+    while (mag > 65536.0 && (!guard || std::isfinite(mag))) {
+        dzx *= inv16;
+        dzy *= inv16;
+        ze += 16;
+        mag = pymax(fabs(dzx), fabs(dzy));
+    }
+    while (mag < inv16 && mag > 0.0 && (!guard || std::isfinite(mag))) {
+        dzx *= 65536.0;
+        dzy *= 65536.0;
+        ze -= 16;
+        mag = pymax(fabs(dzx), fabs(dzy));
+    }
+}
+
+// cpu_compute_perturbation_sample_floatexp with the BLA table, bbsa_skip = 0
+// (math_core/cpu_kernels.py:605-1021).
+Res fexp64_sample(const ap_view& v, const Orbit& o, double si, double sj) {
+    const int type = v.fractal_type;
+    const bool julia = type == 2 || type == 3;
+    const double dcy = (v.ref_py - sj) * v.dy;
+    const double dcx = (si - v.ref_px) * v.dx;
+    double dzx = julia ? dcx : 0.0, dzy = julia ? dcy : 0.0;
+    const double dpx = julia ? 0.0 : dcx, dpy = julia ? 0.0 : dcy;
+    const int E = v.e_scale;
+    const bool strict = v.strict_glitch > 0;
+    const bool has_bla = o.bla && o.levels > 1 && (type == 0 || type == 2 || type == 4);
+    const int ref_len = o.len;
+    double ir, ie;
+    inv_ln(v, ir, ie);
+    int k = 0, glitch = 0, ze = 0;
+    while (k < ref_len) {
+        double rx = o.zr[k], ry = o.zi[k];
+        int eff = ze - E;
+        if (eff >= -50) {
+            double sc = pow2_f64(eff);
+            double zx = rx + dzx * sc, zy = ry + dzy * sc;
+            double mag2 = zx * zx + zy * zy;
+            if (mag2 > kBail) {
+                Res r = escape_res(k, mag2, ir, ie, v.ln_bailout_log);
+                r.glitch = glitch;
+                return r;
+            }
+        }
+        if (strict && k > 15 && glitch_fexp(rx, ry, dzx, dzy, eff)) glitch = 1;
+        if ((type == 0 || type == 2 || type == 4) && has_bla && k < ref_len - 1) {
+            int shift = ze - E;
+            bool can0;
+            double dzm2 = 0.0, dzm = 0.0;
+            if (shift >= -510) {
+                double sc = pow2_f64(shift);
+                dzm2 = (dzx * dzx + dzy * dzy) * (sc * sc);
+                can0 = dzm2 < B(o, 0, k, 4);
+            } else {
+                dzm = sqrt(dzx * dzx + dzy * dzy);
+                double r0 = B(o, 0, k, 4);
+                can0 = r0 > 0.0 && dzm * pow2_f64(std::max(-1020, shift)) < sqrt(r0);
+            }
+            if (can0) {
+                int best = 0;
+                for (int d = o.levels - 1; d > 0; --d) {
+                    if (k + (1 << d) < ref_len) {
+                        double rd = B(o, d, k, 4);
+                        bool ok = shift >= -510
+                                      ? dzm2 < rd
+                                      : (rd > 0.0 && dzm * pow2_f64(std::max(-1020, shift)) < sqrt(rd));
+                        if (ok) {
+                            best = d;
+                            break;
+                        }
+                    }
+                }
+                double ar = B(o, best, k, 0), ai = B(o, best, k, 1);
+                double br = B(o, best, k, 2), bi = B(o, best, k, 3);
+                double scale_c = (ze >= -1020 && ze <= 1020) ? pow2_f64(-ze) : 0.0;
+                double dxs = dpx * scale_c, dys = dpy * scale_c;
+                double ndx = (ar * dzx - ai * dzy) + (br * dxs - bi * dys);
+                dzy = (ar * dzy + ai * dzx) + (br * dys + bi * dxs);
+                dzx = ndx;
+                if (std::isnan(dzx) || std::isnan(dzy) || std::isinf(dzx) || std::isinf(dzy))
+                    return nan_res(k);
+                renorm(dzx, dzy, ze, true);
+                k += 1 << best;
+                continue;
+            }
+        }
+        if (type == 0 || type == 2) {
+            double lx = 2.0 * (rx * dzx - ry * dzy);
+            double ly = 2.0 * (rx * dzy + ry * dzx);
+            int snl = ze - E;
+            if (snl >= -1020) {
+                double s = pow2_f64(snl);
+                lx += (dzx * dzx - dzy * dzy) * s;
+                ly += (2.0 * dzx * dzy) * s;
+            }
+            if (ze >= -1020 && ze <= 1020) {
+                double s = pow2_f64(-ze);
+                lx += dpx * s;
+                ly += dpy * s;
+            }
+            dzx = lx;
+            dzy = ly;
+            renorm(dzx, dzy, ze, true);
+        } else if (type == 1 || type == 3) {
+            int snl = ze - E;
+            double s_nl = snl >= -1020 ? pow2_f64(snl) : 0.0;
+            double s_c = (ze >= -1020 && ze <= 1020) ? pow2_f64(-ze) : 0.0;
+            double sxs = rx >= 0.0 ? 1.0 : -1.0, sys = ry >= 0.0 ? 1.0 : -1.0;
+            double s_esc = eff >= -1020 ? pow2_f64(eff) : 0.0;
+            double adx = dzx * s_esc, ady = dzy * s_esc;
+            double fx, ex, fy, ey;
+            if ((rx + adx) * sxs >= 0.0) {
+                fx = sxs; ex = 0.0;
+            } else {
+                fx = -sxs;
+                ex = eff <= 1020 ? -sxs * 2.0 * rx * pow2_f64(-eff) : 0.0;
+            }
+            if ((ry + ady) * sys >= 0.0) {
+                fy = sys; ey = 0.0;
+            } else {
+                fy = -sys;
+                ey = eff <= 1020 ? -sys * 2.0 * ry * pow2_f64(-eff) : 0.0;
+            }
+            double lx = (2.0 * rx + dzx * s_nl) * dzx - (2.0 * ry + dzy * s_nl) * dzy + dpx * s_c;
+            double dxn = fx * dzx + ex, dyn = fy * dzy + ey;
+            double ly = 2.0 * (fabs(rx) * dyn + fabs(ry) * dxn + dxn * dyn * s_nl) - dpy * s_c;
+            dzx = lx;
+            dzy = ly;
+            // The reference omits the finiteness guard here
+            // (math_core/cpu_kernels.py:868-878) and would loop forever on an
+            // infinite delta; the guard changes no finite case.
+            renorm(dzx, dzy, ze, true);
+        } else {
+            double pr = dzx, pi = dzy, qr = rx, qi = ry;
+            int snl = ze - E;
+            double s_nl = snl >= -1020 ? pow2_f64(snl) : 0.0;
+            for (int e = 0; e < v.gen_n - 1; ++e) {
+                double t1r = pr * rx - pi * ry, t1i = pr * ry + pi * rx;
+                double t2r = qr * dzx - qi * dzy, t2i = qr * dzy + qi * dzx;
+                double t3r = pr * dzx - pi * dzy, t3i = pr * dzy + pi * dzx;
+                pr = t1r + t2r + t3r * s_nl;
+                pi = t1i + t2i + t3i * s_nl;
+                double nq = qr * rx - qi * ry;
+                qi = qr * ry + qi * rx;
+                qr = nq;
+            }
+            double kdr = v.gen_kr * dzx - v.gen_ki * dzy, kdi = v.gen_kr * dzy + v.gen_ki * dzx;
+            double s_c = (ze >= -1020 && ze <= 1020) ? pow2_f64(-ze) : 0.0;
+            dzx = pr + kdr + dpx * s_c;
+            dzy = pi + kdi + dpy * s_c;
+            renorm(dzx, dzy, ze, true);
+        }
+        ++k;
+    }
+    int eff = ze - E;
+    double sfb = eff >= -60 ? pow2_f64(eff) : 0.0;
+    double zx = o.zr[ref_len] + dzx * sfb, zy = o.zi[ref_len] + dzy * sfb;
+    double scfb = E <= 1020 ? pow2_f64(-E) : 0.0;
+    double cx = julia ? v.julia_cx : v.ref_cx + dpx * scfb;
+    double cy = julia ? v.julia_cy : v.ref_cy + dpy * scfb;
+    double mag2 = zx * zx + zy * zy;
+    if (mag2 > kBail) {
+        Res r = escape_res(ref_len, mag2, ir, ie, v.ln_bailout_log);
+        r.glitch = glitch;
+        return r;
+    }
+    if (ref_len < v.max_iter) glitch = 1;
+    return stage2(v, zx, zy, cx, cy, k, glitch);
+}
+
+// ------------------------------------------------ colour --------------------
+double pymod(double a, double b) {
+    double m = fmod(a, b);
+    if (m != 0.0 && ((b < 0.0) != (m < 0.0))) m += b;
+    return m;
+}
+double tri(double ph) { return 1.0 - fabs((ph < 0.0 ? ph + 2.0 : ph) - 1.0); }
+
+// cpu_palette_norm, math_core/cpu_kernels.py:14-78 (sqrt of the float32
+// sample in float32).
+double palette_norm(float s, int scheme, double off, double density, double contrast) {
+    double dens = density <= 0.0 ? 1.0 : density;
+    double cont = contrast <= 0.0 ? 1.0 : contrast;
+    double sd = s;
+    switch (scheme) {
+    case 1: return tri(pymod(log(sd * cont + 1.0) * (0.45 * dens) + off * 2.0, 2.0));
+    case 2: {
+        double t = tri(pymod((double)sqrtf(s) * (0.125 * dens) + off * 2.0, 2.0));
+        double k = 4.0 * cont, td = tanh(0.5 * k);
+        double sc = fabs(td) > 1e-7 ? 0.5 / td : 0.5186574;
+        return 0.5 + tanh((t - 0.5) * k) * sc;
+    }
+    case 3: {
+        double t = tri(pymod((double)sqrtf(s) * (0.125 * dens) + off * 2.0, 2.0));
+        double k = 3.0 * cont, ed = exp(k) - 1.0;
+        double sc = fabs(ed) > 1e-7 ? 1.0 / ed : 0.0523957;
+        return (exp(k * t) - 1.0) * sc;
+    }
+    case 4: {
+        double t = tri(pymod((double)sqrtf(s) * (0.5 * dens) + off * 2.0, 2.0));
+        return cont != 1.0 ? pow(t, cont) : t;
+    }
+    case 5: {
+        double ang = sd * (0.05 * dens) + off * 6.28318530718;
+        return 0.5 + 0.25 * (sin(ang) + sin(ang * (1.61803398875 * cont)));
+    }
+    case 6: {
+        double ang = sd * (0.08 * dens) + off * 6.28318530718;
+        double val = 0.5 * (1.0 - cos(ang));
+        return cont != 1.0 ? pow(val, cont) : val;
+    }
+    case 7: {
+        double ph = sd * (0.05 * dens) + off * 6.28318530718;
+        double b = 0.5 * cont;
+        double w = sin(ph) + b * sin(ph * 2.0 + 1.2) + (b * b) * sin(ph * 4.0 + 2.4) +
+                   (b * b * b) * sin(ph * 8.0 + 3.6);
+        double ns = 1.0 + b + b * b + b * b * b;
+        return 0.5 + (ns > 1e-7 ? 0.5 / ns : 0.26666666666666666) * w;
+    }
+    case 8: {
+        double t = tri(pymod((double)sqrtf(s) * (0.1 * dens) + off * 2.0, 2.0));
+        if (cont != 1.0) t = pow(t, cont);
+        double t2 = t * t;
+        return t2 * t * (t * (t * 6.0 - 15.0) + 10.0);
+    }
+    case 9: {
+        double ang = sd * (0.04 * dens) + (sd * sd) * (0.00008 * dens * cont) + off * 6.28318530718;
+        return 0.5 + 0.5 * sin(ang);
+    }
+    case 10: {
+        double raw = pymod(1.0 - exp(-0.06 * cont * sqrt(sd * dens)) + off, 1.0);
+        return raw >= 0.0 ? raw : raw + 1.0;
+    }
+    default: {
+        double t = tri(pymod((double)sqrtf(s) * (0.125 * dens) + off * 2.0, 2.0));
+        return cont != 1.0 ? pow(t, cont) : t;
+    }
+    }
+}
+
+// cpu_sample_cdf_norm, math_core/cpu_kernels.py:1684-1706.
+double cdf_norm(float s, int max_iter, const float* cdf, int nb, double off, double density,
+                double contrast) {
+    double inv_max = 1.0 / std::max(1.0, (double)max_iter);
+    double pos = ((double)s * inv_max) * (double)(nb - 1);
+    int k = (int)pos;
+    double c;
+    if (k >= nb - 1)
+        c = cdf[nb - 1];
+    else if (k < 0)
+        c = cdf[0];
+    else {
+        double frac = pos - (double)k;
+        c = (double)cdf[k] + frac * (double)(cdf[k + 1] - cdf[k]);
+    }
+    if (contrast != 1.0 && contrast > 0.0) c = pow(c, contrast);
+    double dens = density <= 0.0 ? 1.0 : density;
+    return tri(pymod(c * (2.0 * dens) + off * 2.0, 2.0));
+}
+
+// cpu_sample_lut_lerp, math_core/cpu_kernels.py:81-92.
+void lut_lerp(const float* lut, double norm, double& r, double& g, double& b) {
+    double n = norm < 0.0 ? 0.0 : (norm > 1.0 ? 1.0 : norm);
+    double pos = n * 2047.0;
+    int i0 = (int)pos;
+    int i1 = i0 >= 2047 ? 2047 : i0 + 1;
+    double frac = pos - (double)i0, inv = 1.0 - frac;
+    r = (double)lut[i0 * 3] * inv + (double)lut[i1 * 3] * frac;
+    g = (double)lut[i0 * 3 + 1] * inv + (double)lut[i1 * 3 + 1] * frac;
+    b = (double)lut[i0 * 3 + 2] * inv + (double)lut[i1 * 3 + 2] * frac;
+}
+
+void colour(const ap_view& v, const float* cdf, int nb, const float* lut, float s, double& r,
+            double& g, double& b) {
+    double norm = cdf ? cdf_norm(s, v.max_iter, cdf, nb, v.palette_offset, v.color_density,
+                                 v.color_contrast)
+                      : palette_norm(s, v.scheme_id, v.palette_offset, v.color_density,
+                                     v.color_contrast);
+    lut_lerp(lut, norm, r, g, b);
+}
+
+void put_rgb(void* rgb, int hdr, size_t idx, double r, double g, double b) {
+    if (hdr) {
+        uint16_t* p = (uint16_t*)rgb + idx * 3;
+        p[0] = (uint16_t)(int)(r + 0.5);
+        p[1] = (uint16_t)(int)(g + 0.5);
+        p[2] = (uint16_t)(int)(b + 0.5);
+    } else {
+        uint8_t* p = (uint8_t*)rgb + idx * 3;
+        p[0] = (uint8_t)(int)(r + 0.5);
+        p[1] = (uint8_t)(int)(g + 0.5);
+        p[2] = (uint8_t)(int)(b + 0.5);
+    }
+}
+
+// ------------------------------------------------ pixel driver --------------
+// The per-pixel bodies of cpu_render_*_kernel (math_core/cpu_kernels.py:
+// 1113-1681) and of the GPU unified kernels' mode 0/1/2 branches.
+template <class F>
+int render(const ap_view& v, const ap_outputs& o, bool iter_in_sub, int threads, F sample,
+           long long* iters_out) {
+    const int W = v.width, H = v.height, f = v.factor;
+    long long total = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(static) reduction(+ : total)
+#endif
+    for (int j = 0; j < H; ++j) {
+        for (int i = 0; i < W; ++i) {
+            size_t idx = (size_t)j * W + i;
+            if (v.rebase_pass > 0 && o.mask && o.mask[idx] == 0) continue;
+            if (v.mode == 2 && v.rebase_pass == 0 && o.mask[idx] == 0) continue;
+            if (v.mode == 0) {
+                double off = ((double)f - 1.0) * 0.5;
+                Res r = sample((double)(i * f) + off, (double)(j * f) + off);
+                total += r.iters > 0 ? r.iters : 0;
+                if (o.counts) o.counts[idx] = r.count;
+                if (o.smooth) o.smooth[idx] = r.smooth;
+                if (o.iter_map) o.iter_map[idx] = r.s;
+                if (o.glitch) o.glitch[idx] = r.glitch ? 1 : 0;
+                if (o.rgb) {
+                    double cr = 0, cg = 0, cb = 0;
+                    if (r.s >= 0.0f) colour(v, o.cdf, o.num_bins, o.lut, r.s, cr, cg, cb);
+                    put_rgb(o.rgb, v.is_hdr, idx, cr, cg, cb);
+                }
+                continue;
+            }
+            double ar = 0, ag = 0, ab = 0;
+            int gl = 0;
+            Res r{};
+            for (int sy = 0; sy < f; ++sy)
+                for (int sx = 0; sx < f; ++sx) {
+                    r = sample((double)(i * f + sx), (double)(j * f + sy));
+                    total += r.iters > 0 ? r.iters : 0;
+                    if (r.glitch) ++gl;
+                    if (r.s >= 0.0f) {
+                        double cr, cg, cb;
+                        colour(v, o.cdf, o.num_bins, o.lut, r.s, cr, cg, cb);
+                        ar += cr;
+                        ag += cg;
+                        ab += cb;
+                    }
+                }
+            double inv = 1.0 / (double)(f * f);
+            if (o.rgb) put_rgb(o.rgb, v.is_hdr, idx, ar * inv, ag * inv, ab * inv);
+            if (o.glitch) o.glitch[idx] = gl > 0;
+            if ((iter_in_sub && !o.cdf) || v.rebase_pass > 0) {
+                if (o.iter_map) o.iter_map[idx] = r.s;
+                if (o.counts) o.counts[idx] = r.count;
+                if (o.smooth) o.smooth[idx] = r.smooth;
+            }
+        }
+    }
+    if (iters_out) *iters_out = total;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int or_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+// Single FP64 sample: count, float32 smooth, float64 smooth.
+int or_fp64_sample(const ap_view* v, double x, double y, int* count, float* s, double* smooth) {
+    Res r = fp64_sample(*v, x, y);
+    *count = r.count;
+    *s = r.s;
+    *smooth = r.smooth;
+    return 0;
+}
+
+int or_dd_sample(const ap_view* v, double cxh, double cxl, double cyh, double cyl, int* count,
+                 float* s, double* smooth) {
+    Res r = dd_sample(*v, DD{cxh, cxl}, DD{cyh, cyl});
+    *count = r.count;
+    *s = r.s;
+    *smooth = r.smooth;
+    return 0;
+}
+
+int or_render_fp64(const ap_view* v, const ap_outputs* o, int threads, long long* iters) {
+    const ap_view V = *v;
+    return render(V, *o, false, threads, [&](double si, double sj) {
+        return fp64_sample(V, V.x_min + si * V.dx, V.y_max - sj * V.dy);
+    }, iters);
+}
+
+int or_render_dd(const ap_view* v, const ap_outputs* o, int threads, long long* iters) {
+    const ap_view V = *v;
+    return render(V, *o, false, threads, [&](double si, double sj) {
+        DD cx = dd_add(DD{V.x_min, V.x_min_lo}, dd_mul_f64(DD{V.dx, V.dx_lo}, si));
+        DD cy = dd_sub(DD{V.y_max, V.y_max_lo}, dd_mul_f64(DD{V.dy, V.dy_lo}, sj));
+        return dd_sample(V, cx, cy);
+    }, iters);
+}
+
+int or_render_pert64(const ap_view* v, const double* zr, const double* zi, const double* bla,
+                     int levels, const ap_outputs* o, int threads, long long* iters) {
+    const ap_view V = *v;
+    Orbit ob{zr, zi, V.ref_len, bla, bla ? levels : 0};
+    return render(V, *o, true, threads,
+                  [&](double si, double sj) { return pert64_sample(V, ob, si, sj); }, iters);
+}
+
+int or_render_fexp64(const ap_view* v, const double* zr, const double* zi, const double* bla,
+                     int levels, const ap_outputs* o, int threads, long long* iters) {
+    const ap_view V = *v;
+    Orbit ob{zr, zi, V.ref_len, bla, bla ? levels : 0};
+    return render(V, *o, true, threads,
+                  [&](double si, double sj) { return fexp64_sample(V, ob, si, sj); }, iters);
+}
+
+// _jit_build_bla_table, math_core/high_precision.py:1100-1238; out is
+// float64[levels][ref_len][5], zero-filled here.
+int or_bla_build(const double* zr, const double* zi, int ref_len, int type, double tol, int levels,
+                 int n, double kr, double ki, double* out) {
+    memset(out, 0, sizeof(double) * (size_t)levels * ref_len * 5);
+    if (!(type == 0 || type == 2 || type == 4) || ref_len < 4 || tol <= 0.0) return 0;
+    auto at = [&](int d, int k, int f) -> double& { return out[((size_t)d * ref_len + k) * 5 + f]; };
+    double tol_sq_4 = 4.0 * tol * tol;
+    for (int k = 0; k < ref_len - 1; ++k) {
+        double rx = zr[k], ry = zi[k];
+        double rm2 = rx * rx + ry * ry;
+        if (type == 0 || type == 2) {
+            at(0, k, 0) = 2.0 * rx;
+            at(0, k, 1) = 2.0 * ry;
+            at(0, k, 2) = 1.0;
+            at(0, k, 3) = 0.0;
+            double r_sq = tol_sq_4 * rm2;
+            if (r_sq > 0.25) r_sq = 0.25;
+            if (k == 0 && rm2 < 1e-14) r_sq = 0.0;
+            at(0, k, 4) = r_sq;
+        } else {
+            double pr = 1.0, pi = 0.0;
+            for (int e = 0; e < n - 1; ++e) {
+                double t = pr * rx - pi * ry;
+                pi = pr * ry + pi * rx;
+                pr = t;
+            }
+            double ar = (double)n * pr + kr, ai = (double)n * pi + ki;
+            at(0, k, 0) = ar;
+            at(0, k, 1) = ai;
+            at(0, k, 2) = 1.0;
+            at(0, k, 3) = 0.0;
+            double a_mag = sqrt(ar * ar + ai * ai);
+            double tol_a = tol * a_mag;
+            if (tol_a <= 0.0 || std::isnan(tol_a) || std::isinf(tol_a)) {
+                at(0, k, 4) = 0.0;
+            } else {
+                double r_mag = sqrt(rm2);
+                double r_pow = n == 2 ? 1.0 : (n == 3 ? r_mag : (n == 4 ? rm2 : pow(r_mag, (double)(n - 2))));
+                double b = 0.25 * (double)(n * (n - 1)) * r_pow;
+                double denom = b + sqrt(b * b + tol_a);
+                double x = denom > 1e-300 ? tol_a / denom : 0.0;
+                if (n > 3 && tol_a > 0.0) {
+                    double xm = pow(tol_a, 1.0 / (double)(n - 1));
+                    if (x > xm) x = xm;
+                }
+                if (x > 0.5) x = 0.5;
+                double r_sq = x * x;
+                if (k == 0 && rm2 < 1e-14) r_sq = 0.0;
+                at(0, k, 4) = r_sq;
+            }
+        }
+    }
+    int step = 1;
+    for (int d = 1; d < levels; ++d) {
+        int nxt = step * 2;
+        for (int k = 0; k < ref_len - nxt; ++k) {
+            int k2 = k + step;
+            double r1 = at(d - 1, k, 4), r2 = at(d - 1, k2, 4);
+            if (!(r1 > 0.0 && r2 > 0.0)) continue;
+            double a1r = at(d - 1, k, 0), a1i = at(d - 1, k, 1);
+            double b1r = at(d - 1, k, 2), b1i = at(d - 1, k, 3);
+            double a2r = at(d - 1, k2, 0), a2i = at(d - 1, k2, 1);
+            double b2r = at(d - 1, k2, 2), b2i = at(d - 1, k2, 3);
+            double ar = a2r * a1r - a2i * a1i, ai = a2r * a1i + a2i * a1r;
+            double br = (a2r * b1r - a2i * b1i) + b2r, bi = (a2r * b1i + a2i * b1r) + b2i;
+            double am2 = ar * ar + ai * ai, bm2 = br * br + bi * bi;
+            if (std::isnan(am2) || std::isinf(am2) || std::isnan(bm2) || std::isinf(bm2) ||
+                am2 > 1e150 || bm2 > 1e150)
+                continue;  // entry stays zero
+            at(d, k, 0) = ar;
+            at(d, k, 1) = ai;
+            at(d, k, 2) = br;
+            at(d, k, 3) = bi;
+            double a1m2 = a1r * a1r + a1i * a1i;
+            double cand;
+            if (a1m2 > 1e-300) {
+                double r2s = (0.25 * r2) / a1m2;
+                cand = r1 < r2s ? r1 : r2s;
+            } else {
+                cand = r1;
+            }
+            if (std::isnan(cand) || std::isinf(cand) || cand <= 0.0)
+                at(d, k, 4) = 0.0;
+            else
+                at(d, k, 4) = cand > 0.25 ? 0.25 : cand;
+        }
+        step = nxt;
+    }
+    return 0;
+}
+
+// 8-neighbour edge rule (math_core/cuda_kernels.py:1587-1611; the CPU twin
+// math_core/cpu_kernels.py:1158-1198 gives the same mask).
+int or_edge_mask(const float* it, int w, int h, float thr, uint8_t* mask) {
+    for (int j = 0; j < h; ++j)
+        for (int i = 0; i < w; ++i) {
+            float s = it[(size_t)j * w + i];
+            bool e = false;
+            for (int dj = -1; dj <= 1 && !e; ++dj) {
+                int nj = j + dj;
+                if (nj < 0 || nj >= h) continue;
+                for (int di = -1; di <= 1; ++di) {
+                    int ni = i + di;
+                    if (ni < 0 || ni >= w || (di == 0 && dj == 0)) continue;
+                    float sn = it[(size_t)nj * w + ni];
+                    if ((s < 0.0f) != (sn < 0.0f) || (s >= 0.0f && fabsf(s - sn) > thr)) {
+                        e = true;
+                        break;
+                    }
+                }
+            }
+            mask[(size_t)j * w + i] = e;
+        }
+    return 0;
+}
+
+// cpu_build_histogram, math_core/cpu_kernels.py:1709-1722.
+int or_hist_build(const float* it, long long n, int max_iter, int nb, long long* hist) {
+    memset(hist, 0, sizeof(long long) * nb);
+    double inv_max = 1.0 / std::max(1.0, (double)max_iter);
+    for (long long i = 0; i < n; ++i) {
+        float s = it[i];
+        if (s >= 0.0f) {
+            double pos = ((double)s * inv_max) * (double)(nb - 1);
+            int b = (int)pos;
+            if (b >= nb) b = nb - 1;
+            if (b < 0) b = 0;
+            hist[b] += 1;
+        }
+    }
+    return 0;
+}
+
+// Colour a cached map (histogram apply or palette recolour).
+int or_colorize(const ap_view* v, const float* it, const float* cdf, int nb, const float* lut,
+                void* rgb) {
+    size_t n = (size_t)v->width * v->height;
+    for (size_t i = 0; i < n; ++i) {
+        double r = 0, g = 0, b = 0;
+        if (it[i] >= 0.0f) colour(*v, cdf, nb, lut, it[i], r, g, b);
+        put_rgb(rgb, v->is_hdr, i, r, g, b);
+    }
+    return 0;
+}
+
+}  // extern "C"

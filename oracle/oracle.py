@@ -1,0 +1,158 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+never by the fractura_amd package.  Every function works on host numpy arrays
+and the same ap_view structure the HIP library takes.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from fractura_amd._abi import ApOutputs, ApView
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+LIB_FAST = os.path.join(HERE, "liboracle_fast.so")
+
+_libs = {}
+
+
+def build():
+    subprocess.run(["make", "-B", "-C", HERE], check=True, stdout=subprocess.DEVNULL)
+
+
+def load(fast=False):
+    path = LIB_FAST if fast else LIB
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
+        build()
+    lib = C.CDLL(path)
+    V, O, P, I, D = C.POINTER(ApView), C.POINTER(ApOutputs), C.c_void_p, C.c_int, C.c_double
+    LL = C.POINTER(C.c_longlong)
+    sig = {
+        "or_num_threads": (I, []),
+        "or_fp64_sample": (I, [V, D, D, C.POINTER(I), C.POINTER(C.c_float), C.POINTER(D)]),
+        "or_dd_sample": (I, [V, D, D, D, D, C.POINTER(I), C.POINTER(C.c_float), C.POINTER(D)]),
+        "or_render_fp64": (I, [V, O, I, LL]),
+        "or_render_dd": (I, [V, O, I, LL]),
+        "or_render_pert64": (I, [V, P, P, P, I, O, I, LL]),
+        "or_render_fexp64": (I, [V, P, P, P, I, O, I, LL]),
+        "or_bla_build": (I, [P, P, I, I, D, I, I, D, D, P]),
+        "or_edge_mask": (I, [P, I, I, C.c_float, P]),
+        "or_hist_build": (I, [P, C.c_longlong, I, I, P]),
+        "or_colorize": (I, [V, P, P, I, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _libs[path] = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Maps:
+    """Host outputs of one oracle pass."""
+
+    def __init__(self, view, rgb=True, glitch=True):
+        H, W = view.height, view.width
+        self.counts = np.full((H, W), -7, np.int32)
+        self.smooth = np.full((H, W), np.nan, np.float64)
+        self.iter_map = np.full((H, W), np.nan, np.float32)
+        self.glitch = np.zeros((H, W), np.uint8) if glitch else None
+        self.rgb = np.zeros((H, W, 3), np.uint16 if view.is_hdr else np.uint8) if rgb else None
+        self.iterations = 0
+
+    def outputs(self, lut=None, mask=None, cdf=None):
+        o = ApOutputs()
+        o.counts, o.smooth, o.iter_map = _p(self.counts), _p(self.smooth), _p(self.iter_map)
+        o.glitch = _p(self.glitch)
+        o.rgb = _p(self.rgb) if lut is not None else None
+        o.lut = _p(lut)
+        o.mask = _p(mask)
+        if cdf is not None:
+            o.cdf = _p(cdf)
+            o.num_bins = len(cdf)
+        return o
+
+
+def render(engine, view, lut=None, mask=None, cdf=None, orbit=None, bla=None, levels=0,
+           threads=0, fast=False, maps=None):
+    """Run one oracle pass; returns Maps (pass `maps` to update in place)."""
+    lib = load(fast)
+    m = maps if maps is not None else Maps(view, rgb=lut is not None)
+    keep = (lut, mask, cdf)
+    o = m.outputs(*keep)
+    it = C.c_longlong(0)
+    if engine == "fp64":
+        lib.or_render_fp64(C.byref(view), C.byref(o), threads, C.byref(it))
+    elif engine == "dd":
+        lib.or_render_dd(C.byref(view), C.byref(o), threads, C.byref(it))
+    elif engine in ("pert64", "fexp64"):
+        re, im = orbit
+        re = np.ascontiguousarray(re, np.float64)
+        im = np.ascontiguousarray(im, np.float64)
+        fn = lib.or_render_pert64 if engine == "pert64" else lib.or_render_fexp64
+        fn(C.byref(view), _p(re), _p(im), _p(bla), levels if bla is not None else 0, C.byref(o),
+           threads, C.byref(it))
+    else:
+        raise ValueError(engine)
+    m.iterations = it.value
+    return m
+
+
+def fp64_sample(view, x, y):
+    c, s, sm = C.c_int(), C.c_float(), C.c_double()
+    load().or_fp64_sample(C.byref(view), x, y, C.byref(c), C.byref(s), C.byref(sm))
+    return c.value, s.value, sm.value
+
+
+def dd_sample(view, cx, cy):
+    c, s, sm = C.c_int(), C.c_float(), C.c_double()
+    load().or_dd_sample(C.byref(view), cx[0], cx[1], cy[0], cy[1], C.byref(c), C.byref(s), C.byref(sm))
+    return c.value, s.value, sm.value
+
+
+def bla_build(re, im, ref_len, fractal_type=0, tol=1e-6, levels=16, n=3, kr=0.25, ki=1.0):
+    out = np.zeros((levels, ref_len, 5), np.float64)
+    re = np.ascontiguousarray(re, np.float64)
+    im = np.ascontiguousarray(im, np.float64)
+    load().or_bla_build(_p(re), _p(im), ref_len, fractal_type, tol, levels, n, kr, ki, _p(out))
+    return out
+
+
+def edge_mask(iter_map, threshold=0.35):
+    it = np.ascontiguousarray(iter_map, np.float32)
+    m = np.zeros(it.shape, np.uint8)
+    load().or_edge_mask(_p(it), it.shape[1], it.shape[0], C.c_float(np.float32(threshold)), _p(m))
+    return m
+
+
+def hist_build(iter_map, max_iter, num_bins=65536):
+    it = np.ascontiguousarray(iter_map, np.float32).ravel()
+    h = np.zeros(num_bins, np.int64)
+    load().or_hist_build(_p(it), it.size, max_iter, num_bins, _p(h))
+    return h
+
+
+def cdf_from_hist(hist):
+    """np.cumsum / total in float64 -> float32, linear ramp when empty
+    (math_core/engine.py:639-643)."""
+    c = np.cumsum(hist, dtype=np.float64)
+    tot = c[-1]
+    if tot > 0:
+        return (c / tot).astype(np.float32)
+    return np.linspace(0.0, 1.0, len(hist), dtype=np.float32)
+
+
+def colorize(view, iter_map, lut, cdf=None):
+    it = np.ascontiguousarray(iter_map, np.float32)
+    rgb = np.zeros(it.shape + (3,), np.uint16 if view.is_hdr else np.uint8)
+    load().or_colorize(C.byref(view), _p(it), _p(cdf), 0 if cdf is None else len(cdf), _p(lut), _p(rgb))
+    return rgb
