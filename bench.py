@@ -310,12 +310,31 @@ def run_fp64(args, torch):
     }
 
 
+def run_copy(args, torch):
+    """Plain device copy of a known size: the calibration run for the HBM
+    byte counters (FETCH_SIZE / WRITE_SIZE) of the profiler."""
+    nbytes = 256 << 20
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(args.warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    return {"metric": "copy GB/s (profiler calibration)", "value": 2 * nbytes / dt / 1e9,
+            "unit": "GB/s", "ms_per_step": dt * 1e3, "dtype": "f32",
+            "config": {"workload": f"device copy of {nbytes} bytes"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["deep", "fp64"], default="deep")
+    ap.add_argument("--workload", choices=["deep", "fp64", "copy"], default="deep")
     ap.add_argument("--size", type=int, nargs=2, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -329,7 +348,7 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
-    res = run_deep(args, torch) if args.workload == "deep" else run_fp64(args, torch)
+    res = {"deep": run_deep, "fp64": run_fp64, "copy": run_copy}[args.workload](args, torch)
     line = {"metric": res.pop("metric"), "value": res.pop("value"), "unit": res.pop("unit"),
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": res.pop("ms_per_step"), "higher_is_better": True, "scaling": "weak",

@@ -41,11 +41,17 @@ __device__ inline Sample interior(int count, long long iters) {
     return r;
 }
 
+// ln(ln 256) = ln(0.5 ln 65536), correctly rounded.  The reference passes
+// 1.7129177284488854 (config.py:26), 1.09e-5 below it; the float32 map keeps
+// the reference's constant, the float64 smooth value uses this one.
+constexpr double kLnLn256 = 1.7129286210981716;
+
 // Escape epilogue of every reference sample evaluator
 // (math_core/cpu_kernels.py:246-249): the float32 value is
 // f32(k+1) - f32((ln(0.5 ln|z|^2) - LN_BAILOUT_LOG) * inv_ln) with float32
-// subtraction; the float64 value keeps the same chain in double and uses the
-// exact 1/ln n for type 4 (the reference rounds it to float32).
+// subtraction; the float64 value is k+1 - log2(ln|z| / ln 256) in double, with
+// the exact ln(ln 256) and the exact 1/ln n for type 4 (the reference rounds
+// 1/ln n to float32).
 __device__ inline Sample escaped(int k, double mag2, double inv_ln_ref,
                                  double inv_ln_exact, double ln_bailout_log) {
     Sample r;
@@ -56,9 +62,9 @@ __device__ inline Sample escaped(int k, double mag2, double inv_ln_ref,
     r.bla = 0;
     double log_mag = log(mag2);
     if (log_mag > 1.0) {
-        double nu = log(0.5 * log_mag) - ln_bailout_log;
-        r.s = (float)(k + 1.0) - (float)(nu * inv_ln_ref);
-        r.smooth = (double)(k + 1) - nu * inv_ln_exact;
+        double ll = log(0.5 * log_mag);
+        r.s = (float)(k + 1.0) - (float)((ll - ln_bailout_log) * inv_ln_ref);
+        r.smooth = (double)(k + 1) - (ll - kLnLn256) * inv_ln_exact;
     } else {
         r.s = (float)k;
         r.smooth = (double)k;

@@ -196,11 +196,17 @@ __device__ Sample pert64_sample(const ap_view& v, const PertData& D, double si, 
         if (has_bla && k < ref_len - 1) {
             const double dz_mag2 = dzx * dzx + dzy * dzy;
             const size_t row = (size_t)k * D.bla.levels;
-            if (dz_mag2 < D.bla.r2[row]) {
-                int max_d = D.bla.levels - 1;
-                if (dz_mag2 == 0.0 && (dzx != 0.0 || dzy != 0.0)) max_d = max_d < 8 ? max_d : 8;
-                int hi = min(max_d, level_limit_by_length(k, ref_len));
-                int best = highest_valid_level(hi, [&](int d) { return dz_mag2 < D.bla.r2[row + d]; });
+            int max_d = D.bla.levels - 1;
+            if (dz_mag2 == 0.0 && (dzx != 0.0 || dzy != 0.0)) max_d = max_d < 8 ? max_d : 8;
+            const int hi = min(max_d, level_limit_by_length(k, ref_len));
+            // Level 0 and the top level are read together; the search runs
+            // only when the top level is not valid.
+            const double r0 = D.bla.r2[row], rtop = D.bla.r2[row + hi];
+            if (dz_mag2 < r0) {
+                int best = dz_mag2 < rtop ? hi
+                                          : highest_valid_level(hi - 1, [&](int d) {
+                                                return dz_mag2 < D.bla.r2[row + d];
+                                            });
                 double4 c = D.bla.coef[row + best];
                 double ndx = (c.x * dzx - c.y * dzy) + (c.z * dpx - c.w * dpy);
                 dzy = (c.x * dzy + c.y * dzx) + (c.z * dpy + c.w * dpx);
@@ -306,24 +312,33 @@ __device__ Sample fexp64_sample(const ap_view& v, const PertData& D, double si, 
             if (has_bla && k < ref_len - 1) {
                 const int shift = z_exp - E;
                 const size_t row = (size_t)k * D.bla.levels;
-                bool can0;
+                const int hi = min(D.bla.levels - 1, level_limit_by_length(k, ref_len));
+                bool can0, cantop;
                 double dz_mag2 = 0.0, dz_mag = 0.0, tiny = 0.0;
+                // Level 0 and the top level are read together; the search runs
+                // only when the top level is not valid.
                 if (shift >= -510) {
                     double sc = pow2_f64(shift);
                     dz_mag2 = (dzx * dzx + dzy * dzy) * (sc * sc);
-                    can0 = dz_mag2 < D.bla.r2[row];
+                    const double r0 = D.bla.r2[row], rtop = D.bla.r2[row + hi];
+                    can0 = dz_mag2 < r0;
+                    cantop = dz_mag2 < rtop;
                 } else {
                     dz_mag = sqrt(dzx * dzx + dzy * dzy);
                     tiny = dz_mag * pow2_f64(shift > -1020 ? shift : -1020);
-                    can0 = D.bla.r2[row] > 0.0 && tiny < D.bla.r1[row];
+                    const double r0 = D.bla.r2[row], rtop = D.bla.r2[row + hi];
+                    const double s0 = D.bla.r1[row], stop = D.bla.r1[row + hi];
+                    can0 = r0 > 0.0 && tiny < s0;
+                    cantop = rtop > 0.0 && tiny < stop;
                 }
                 if (can0) {
-                    int hi = min(D.bla.levels - 1, level_limit_by_length(k, ref_len));
                     int best;
-                    if (shift >= -510)
-                        best = highest_valid_level(hi, [&](int d) { return dz_mag2 < D.bla.r2[row + d]; });
+                    if (cantop)
+                        best = hi;
+                    else if (shift >= -510)
+                        best = highest_valid_level(hi - 1, [&](int d) { return dz_mag2 < D.bla.r2[row + d]; });
                     else
-                        best = highest_valid_level(hi, [&](int d) {
+                        best = highest_valid_level(hi - 1, [&](int d) {
                             return D.bla.r2[row + d] > 0.0 && tiny < D.bla.r1[row + d];
                         });
                     double4 c = D.bla.coef[row + best];
@@ -469,6 +484,92 @@ __device__ inline void renorm32(float& dzx, float& dzy, int& z_exp) {
     }
 }
 
+// Hand-off point of the FP32 loop: once |dz| >= 2^kTail32 the delta continues
+// in FP64 with plain steps.  Near escape a chaotic excursion amplifies the
+// error already in dz by the product of |2z| over the excursion; a 24-bit
+// mantissa, and the 1e-6 error of BLA leaps taken that late, do not survive
+// it.  Below 2^-32 the delta is a linear correction whose relative error does
+// not grow, so the FP32 loop and its BLA leaps cover all the deep iterations.
+// (Measured on seeded deep views: p99 smooth error against exact FP64
+// perturbation 1.4e-5..1.8e-5, versus 3e-4..4e-4 for the FP64-mantissa loop
+// with BLA to the end.)
+constexpr int kTail32 = -32;
+
+// FP64 perturbation continuation of the FP32 loop from iteration k, with the
+// delta now in actual units (dc * 2^-E underflows to 0 beyond FP64 range,
+// where it is negligible next to |dz| >= 2^kTail32).
+template <int T>
+__device__ Sample fexp32_tail(const ap_view& v, const Fexp32Data& D, const PertConsts& pc,
+                              double dzx, double dzy, double dcx_d, double dcy_d, int k,
+                              int glitch, int plain, int nbla) {
+    const bool julia = (T == 2 || T == 3);
+    const int ref_len = v.ref_len;
+    const bool strict = v.strict_glitch > 0;
+    const double sc_e = v.e_scale <= 1074 ? ldexp(1.0, -v.e_scale) : 0.0;
+    const double dpx = julia ? 0.0 : dcx_d * sc_e, dpy = julia ? 0.0 : dcy_d * sc_e;
+    const int n = v.gen_n;
+    while (k < ref_len) {
+        const double rx = D.zr[k], ry = D.zi[k];
+        const double zx = rx + dzx, zy = ry + dzy;
+        const double mag2 = zx * zx + zy * zy;
+        if (mag2 > kBailout2) {
+            Sample r = escaped(k, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
+            r.glitch = glitch;
+            r.plain = plain;
+            r.bla = nbla;
+            return r;
+        }
+        if (strict && k > 15) {
+            double r_mag2 = rx * rx + ry * ry;
+            if (mag2 < 1e-6 * r_mag2 && r_mag2 > 1e-8) glitch = 1;
+        }
+        if (T == 0 || T == 2) {
+            double ndx = (2.0 * rx + dzx) * dzx - (2.0 * ry + dzy) * dzy + dpx;
+            dzy = 2.0 * ((rx + dzx) * dzy + ry * dzx) + dpy;
+            dzx = ndx;
+        } else if (T == 1 || T == 3) {
+            double ndx = (2.0 * rx + dzx) * dzx - (2.0 * ry + dzy) * dzy + dpx;
+            double sxs = rx >= 0.0 ? 1.0 : -1.0;
+            double sys = ry >= 0.0 ? 1.0 : -1.0;
+            double ddx = (rx + dzx) * sxs >= 0.0 ? sxs * dzx : -sxs * (2.0 * rx + dzx);
+            double ddy = (ry + dzy) * sys >= 0.0 ? sys * dzy : -sys * (2.0 * ry + dzy);
+            dzy = 2.0 * (fabs(rx) * ddy + fabs(ry) * ddx + ddx * ddy) - dpy;
+            dzx = ndx;
+        } else {
+            double pr = dzx, pi = dzy, qr = rx, qi = ry;
+            for (int e = 0; e < n - 1; ++e) {
+                double t1r = pr * rx - pi * ry, t1i = pr * ry + pi * rx;
+                double sr = pr + qr, sim = pi + qi;
+                double t2r = sr * dzx - sim * dzy, t2i = sr * dzy + sim * dzx;
+                pr = t1r + t2r;
+                pi = t1i + t2i;
+                double nq = qr * rx - qi * ry;
+                qi = qr * ry + qi * rx;
+                qr = nq;
+            }
+            double kdr = v.gen_kr * dzx - v.gen_ki * dzy, kdi = v.gen_kr * dzy + v.gen_ki * dzx;
+            dzx = pr + kdr + dpx;
+            dzy = pi + kdi + dpy;
+        }
+        ++k;
+        ++plain;
+    }
+    const double zx = D.zr[ref_len] + dzx, zy = D.zi[ref_len] + dzy;
+    const double scfb = v.e_scale <= 1020 ? pow2_f64(-v.e_scale) : 0.0;
+    const double cx = julia ? v.julia_cx : v.ref_cx + dcx_d * scfb;
+    const double cy = julia ? v.julia_cy : v.ref_cy + dcy_d * scfb;
+    const double mag2 = zx * zx + zy * zy;
+    if (mag2 > kBailout2) {
+        Sample r = escaped(ref_len, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
+        r.glitch = glitch;
+        r.plain = plain;
+        r.bla = nbla;
+        return r;
+    }
+    if (ref_len < v.max_iter) glitch = 1;
+    return stage2<T>(v, pc, zx, zy, cx, cy, k, glitch, plain, nbla);
+}
+
 template <int T>
 __device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si, double sj) {
     const bool julia = (T == 2 || T == 3);
@@ -490,18 +591,12 @@ __device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si
 
     while (k < ref_len) {
         const int eff = z_exp - E;
-        if (eff >= -50) {
-            // Near bailout: the test and the smooth value in FP64.
-            double sc = ldexp(1.0, eff);
-            double zx = D.zr[k] + (double)dzx * sc, zy = D.zi[k] + (double)dzy * sc;
-            double mag2 = zx * zx + zy * zy;
-            if (mag2 > kBailout2) {
-                Sample r = escaped(k, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
-                r.glitch = glitch;
-                r.plain = plain;
-                r.bla = nbla;
-                return r;
-            }
+        // |dz| = max|dz_m| 2^eff >= 2^kTail32: continue in FP64.
+        if (eff >= kTail32 - 40) {
+            float m = fmaxf(fabsf(dzx), fabsf(dzy));
+            if (m > 0.0f && eff + ilogbf(m) >= kTail32)
+                return fexp32_tail<T>(v, D, pc, ldexp((double)dzx, eff), ldexp((double)dzy, eff),
+                                      dcx_d, dcy_d, k, glitch, plain, nbla);
         }
         const float2 Z = D.zf[k];
         if (strict && k > 15 && eff >= -40) {
@@ -520,9 +615,15 @@ __device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si
                 // log2 |dz| in actual units.
                 float m2 = dzx * dzx + dzy * dzy;
                 float ldz = m2 > 0.0f ? 0.5f * __log2f(m2) + (float)(z_exp - E) : -INFINITY;
-                if (ldz < D.bla.lr[row]) {
-                    int hi = min(D.bla.levels - 1, level_limit_by_length(k, ref_len));
-                    int best = highest_valid_level(hi, [&](int d) { return ldz < D.bla.lr[row + d]; });
+                const int hi = min(D.bla.levels - 1, level_limit_by_length(k, ref_len));
+                // Level 0 and the top level are read together; the search runs
+                // only when the top level is not valid.
+                const float l0 = D.bla.lr[row], ltop = D.bla.lr[row + hi];
+                if (ldz < l0) {
+                    int best = ldz < ltop ? hi
+                                          : highest_valid_level(hi - 1, [&](int d) {
+                                                return ldz < D.bla.lr[row + d];
+                                            });
                     const Coef32 c = D.bla.c32[row + best];
                     // dz' = A dz + B dc: exponents e1 = z_exp + ae (A dz) and
                     // e2 = be (B dc, dc in units of 2^-E); the larger one becomes
@@ -581,29 +682,18 @@ __device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si
                 qi = qr * Z.y + qi * Z.x;
                 qr = nq;
             }
-            dzx = pr + (kr * dzx - ki * dzy) + cxs;
+            float ndx = pr + (kr * dzx - ki * dzy) + cxs;
             dzy = pi + (kr * dzy + ki * dzx) + cys;
+            dzx = ndx;
         }
         renorm32(dzx, dzy, z_exp);
         ++k;
         ++plain;
     }
-    const int eff = z_exp - E;
-    const double sfb = eff >= -60 ? ldexp(1.0, eff) : 0.0;
-    const double zx = D.zr[ref_len] + (double)dzx * sfb, zy = D.zi[ref_len] + (double)dzy * sfb;
-    const double scfb = E <= 1020 ? pow2_f64(-E) : 0.0;
-    const double cx = julia ? v.julia_cx : v.ref_cx + dcx_d * scfb;
-    const double cy = julia ? v.julia_cy : v.ref_cy + dcy_d * scfb;
-    const double mag2 = zx * zx + zy * zy;
-    if (mag2 > kBailout2) {
-        Sample r = escaped(ref_len, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
-        r.glitch = glitch;
-        r.plain = plain;
-        r.bla = nbla;
-        return r;
-    }
-    if (ref_len < v.max_iter) glitch = 1;
-    return stage2<T>(v, pc, zx, zy, cx, cy, k, glitch, plain, nbla);
+    // The reference escaped with the delta still tiny: Stage 2 from
+    // Z_ref_len + dz in FP64.
+    return fexp32_tail<T>(v, D, pc, ldexp((double)dzx, z_exp - E), ldexp((double)dzy, z_exp - E),
+                          dcx_d, dcy_d, k, glitch, plain, nbla);
 }
 
 template <int T>

@@ -87,7 +87,7 @@ def plan_view(target_w, target_h, max_iter, center_x_dec, center_y_dec, plot_wid
               gen_ki=1.0, precision_mode="1e-300 (Perturbation)", bbsa_tol=1e-4, bbsa_order=4,
               glitch_mode="Off (Single-Ref)", color_scheme_id=0, is_hdr=False, palette_offset=0.0,
               color_density=1.0, color_contrast=1.0, force_floatexp=False, fexp_mantissa=64,
-              compute_orbit=True, probe=True):
+              compute_orbit=True, probe=True, engine_override=None):
     """Host-side constants of one frame: engine, grid doubles, reference orbit.
 
     The grid follows math_core/engine.py:1386-1421: h = w*H/W in Decimal,
@@ -101,7 +101,10 @@ def plan_view(target_w, target_h, max_iter, center_x_dec, center_y_dec, plot_wid
     plot_width_dec = Decimal(str(plot_width_dec))
     W, H = int(target_w), int(target_h)
     f = max(1, int(ssaa_factor))
-    engine = select_engine(plot_width_dec, precision_mode, force_floatexp, fexp_mantissa)
+    engine = engine_override or select_engine(plot_width_dec, precision_mode, force_floatexp,
+                                              fexp_mantissa)
+    if engine not in ENGINES:
+        raise ValueError(f"unknown engine {engine!r}")
     v = _abi.ApView()
     v.width, v.height, v.factor, v.max_iter = W, H, f, int(max_iter)
     v.fractal_type, v.gen_n = int(fractal_type), int(gen_n)

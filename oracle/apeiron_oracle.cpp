@@ -24,6 +24,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "apeiron_hip.h"
 
@@ -53,9 +54,11 @@ Res escape_res(int k, double mag2, double inv_ref, double inv_exact, double ln_b
     r.iters = k;
     double lm = log(mag2);
     if (lm > 1.0) {
-        double nu = log(0.5 * lm) - ln_bail;
-        r.s = (float)(k + 1.0) - (float)(nu * inv_ref);
-        r.smooth = (double)(k + 1) - nu * inv_exact;
+        // float32 map: the reference's LN_BAILOUT_LOG; float64 value: exact
+        // ln(ln 256) (the reference constant is 1.09e-5 low).
+        double ll = log(0.5 * lm);
+        r.s = (float)(k + 1.0) - (float)((ll - ln_bail) * inv_ref);
+        r.smooth = (double)(k + 1) - (ll - 1.7129286210981716) * inv_exact;
     } else {
         r.s = (float)k;
         r.smooth = (double)k;
@@ -543,6 +546,204 @@ Res fexp64_sample(const ap_view& v, const Orbit& o, double si, double sj) {
     return stage2(v, zx, zy, cx, cy, k, glitch);
 }
 
+// ------------------------------------------------ FloatExp FP32 -------------
+// CPU restatement of this build's FP32-mantissa FloatExp loop (it has no
+// reference counterpart; the reference FloatExp mantissa is FP64).  It pins
+// the device kernel's arithmetic; its accuracy is judged against the FP64
+// loop above and against mpmath.  The FP32 BLA forms are derived from the
+// reference-layout table exactly as the device derives them.
+struct Coef32 {
+    float ar, ai, br, bi;
+    int ae, be;
+};
+
+struct Orbit32 {
+    const Orbit* o;
+    std::vector<float> zfx, zfy, lr;
+    std::vector<Coef32> c32;
+};
+
+void renorm32(float& dzx, float& dzy, int& ze) {
+    float m = std::max(fabsf(dzx), fabsf(dzy));
+    if ((m > 4294967296.0f || m < 2.3283064365386963e-10f) && m > 0.0f && std::isfinite(m)) {
+        int e = ilogbf(m);
+        dzx = ldexpf(dzx, -e);
+        dzy = ldexpf(dzy, -e);
+        ze += e;
+    }
+}
+
+// Hand-off of the FP32 loop to FP64 once |dz| >= 2^kTail32 (see the device
+// code for the reason).
+const int kTail32 = -32;
+
+Res fexp32_tail(const ap_view& v, const Orbit& o, double dzx, double dzy, double dcx_d,
+                double dcy_d, int k, int glitch) {
+    const int type = v.fractal_type;
+    const bool julia = type == 2 || type == 3;
+    const int ref_len = o.len;
+    const bool strict = v.strict_glitch > 0;
+    const double sc_e = v.e_scale <= 1074 ? ldexp(1.0, -v.e_scale) : 0.0;
+    const double dpx = julia ? 0.0 : dcx_d * sc_e, dpy = julia ? 0.0 : dcy_d * sc_e;
+    double ir, ie;
+    inv_ln(v, ir, ie);
+    while (k < ref_len) {
+        double rx = o.zr[k], ry = o.zi[k];
+        double zx = rx + dzx, zy = ry + dzy;
+        double mag2 = zx * zx + zy * zy;
+        if (mag2 > kBail) {
+            Res r = escape_res(k, mag2, ir, ie, v.ln_bailout_log);
+            r.glitch = glitch;
+            return r;
+        }
+        if (strict && k > 15) {
+            double rm2 = rx * rx + ry * ry;
+            if (mag2 < 1e-6 * rm2 && rm2 > 1e-8) glitch = 1;
+        }
+        if (type == 0 || type == 2) {
+            double ndx = (2.0 * rx + dzx) * dzx - (2.0 * ry + dzy) * dzy + dpx;
+            dzy = 2.0 * ((rx + dzx) * dzy + ry * dzx) + dpy;
+            dzx = ndx;
+        } else if (type == 1 || type == 3) {
+            double ndx = (2.0 * rx + dzx) * dzx - (2.0 * ry + dzy) * dzy + dpx;
+            double sxs = rx >= 0.0 ? 1.0 : -1.0, sys = ry >= 0.0 ? 1.0 : -1.0;
+            double ddx = (rx + dzx) * sxs >= 0.0 ? sxs * dzx : -sxs * (2.0 * rx + dzx);
+            double ddy = (ry + dzy) * sys >= 0.0 ? sys * dzy : -sys * (2.0 * ry + dzy);
+            dzy = 2.0 * (fabs(rx) * ddy + fabs(ry) * ddx + ddx * ddy) - dpy;
+            dzx = ndx;
+        } else {
+            double pr = dzx, pi = dzy, qr = rx, qi = ry;
+            for (int e = 0; e < v.gen_n - 1; ++e) {
+                double t1r = pr * rx - pi * ry, t1i = pr * ry + pi * rx;
+                double sr = pr + qr, sim = pi + qi;
+                double t2r = sr * dzx - sim * dzy, t2i = sr * dzy + sim * dzx;
+                pr = t1r + t2r;
+                pi = t1i + t2i;
+                double nq = qr * rx - qi * ry;
+                qi = qr * ry + qi * rx;
+                qr = nq;
+            }
+            double kdr = v.gen_kr * dzx - v.gen_ki * dzy, kdi = v.gen_kr * dzy + v.gen_ki * dzx;
+            dzx = pr + kdr + dpx;
+            dzy = pi + kdi + dpy;
+        }
+        ++k;
+    }
+    double zx = o.zr[ref_len] + dzx, zy = o.zi[ref_len] + dzy;
+    double scfb = v.e_scale <= 1020 ? pow2_f64(-v.e_scale) : 0.0;
+    double cx = julia ? v.julia_cx : v.ref_cx + dcx_d * scfb;
+    double cy = julia ? v.julia_cy : v.ref_cy + dcy_d * scfb;
+    double mag2 = zx * zx + zy * zy;
+    if (mag2 > kBail) {
+        Res r = escape_res(ref_len, mag2, ir, ie, v.ln_bailout_log);
+        r.glitch = glitch;
+        return r;
+    }
+    if (ref_len < v.max_iter) glitch = 1;
+    return stage2(v, zx, zy, cx, cy, k, glitch);
+}
+
+Res fexp32_sample(const ap_view& v, const Orbit32& O, double si, double sj) {
+    const Orbit& o = *O.o;
+    const int type = v.fractal_type;
+    const bool julia = type == 2 || type == 3;
+    const double dcy_d = (v.ref_py - sj) * v.dy;
+    const double dcx_d = (si - v.ref_px) * v.dx;
+    const float dcx = (float)dcx_d, dcy = (float)dcy_d;
+    float dzx = julia ? dcx : 0.0f, dzy = julia ? dcy : 0.0f;
+    const float dpx = julia ? 0.0f : dcx, dpy = julia ? 0.0f : dcy;
+    int ze = 0;
+    renorm32(dzx, dzy, ze);
+    const int E = v.e_scale;
+    const int ref_len = o.len;
+    const int L = o.levels;
+    const bool strict = v.strict_glitch > 0;
+    const bool has_bla = o.bla && L > 1 && (type == 0 || type == 2 || type == 4);
+    const float kr = (float)v.gen_kr, ki = (float)v.gen_ki;
+    int k = 0, glitch = 0;
+    while (k < ref_len) {
+        const int eff = ze - E;
+        if (eff >= kTail32 - 40) {
+            float m = std::max(fabsf(dzx), fabsf(dzy));
+            if (m > 0.0f && eff + ilogbf(m) >= kTail32)
+                return fexp32_tail(v, o, ldexp((double)dzx, eff), ldexp((double)dzy, eff), dcx_d,
+                                   dcy_d, k, glitch);
+        }
+        const float Zx = O.zfx[k], Zy = O.zfy[k];
+        if (strict && k > 15 && eff >= -40) {
+            float rmax = std::max(fabsf(Zx), fabsf(Zy));
+            if (rmax >= 1e-4f && (dzx != 0.0f || dzy != 0.0f)) {
+                float zx = Zx + ldexpf(dzx, eff), zy = Zy + ldexpf(dzy, eff);
+                if (zx * zx + zy * zy < 1e-6f * (Zx * Zx + Zy * Zy)) glitch = 1;
+            }
+        }
+        if (has_bla && k < ref_len - 1) {
+            float m2 = dzx * dzx + dzy * dzy;
+            float ldz = m2 > 0.0f ? 0.5f * log2f(m2) + (float)(ze - E) : -INFINITY;
+            if (ldz < O.lr[(size_t)k * L]) {
+                int best = 0;
+                for (int d = L - 1; d > 0; --d)
+                    if (k + (1 << d) < ref_len && ldz < O.lr[(size_t)k * L + d]) {
+                        best = d;
+                        break;
+                    }
+                const Coef32 c = O.c32[(size_t)k * L + best];
+                float tr = c.ar * dzx - c.ai * dzy, ti = c.ar * dzy + c.ai * dzx;
+                float ur = c.br * dpx - c.bi * dpy, ui = c.br * dpy + c.bi * dpx;
+                int e1 = ze + c.ae, e2 = c.be;
+                if (tr == 0.0f && ti == 0.0f) e1 = e2;
+                if (ur == 0.0f && ui == 0.0f) e2 = e1;
+                int ne = std::max(e1, e2);
+                dzx = ldexpf(tr, e1 - ne) + ldexpf(ur, e2 - ne);
+                dzy = ldexpf(ti, e1 - ne) + ldexpf(ui, e2 - ne);
+                ze = ne;
+                if (!std::isfinite(dzx) || !std::isfinite(dzy)) return nan_res(k);
+                renorm32(dzx, dzy, ze);
+                k += 1 << best;
+                continue;
+            }
+        }
+        const float s_nl = ldexpf(1.0f, std::max(ze - E, -200));
+        const float cxs = ldexpf(dpx, std::max(-ze, -200)), cys = ldexpf(dpy, std::max(-ze, -200));
+        if (type == 0 || type == 2) {
+            float tx = dzx * s_nl, ty = dzy * s_nl;
+            float lx = 2.0f * (Zx * dzx - Zy * dzy) + (tx * dzx - ty * dzy) + cxs;
+            float ly = 2.0f * (Zx * dzy + Zy * dzx) + 2.0f * tx * dzy + cys;
+            dzx = lx;
+            dzy = ly;
+        } else if (type == 1 || type == 3) {
+            float sxs = Zx >= 0.0f ? 1.0f : -1.0f, sys = Zy >= 0.0f ? 1.0f : -1.0f;
+            float ax = ldexpf(dzx, std::max(ze - E, -200)), ay = ldexpf(dzy, std::max(ze - E, -200));
+            float back = ldexpf(1.0f, std::min(E - ze, 200));
+            float ddx = (Zx + ax) * sxs >= 0.0f ? sxs * dzx : -sxs * (2.0f * Zx * back + dzx);
+            float ddy = (Zy + ay) * sys >= 0.0f ? sys * dzy : -sys * (2.0f * Zy * back + dzy);
+            float lx = (2.0f * Zx + dzx * s_nl) * dzx - (2.0f * Zy + dzy * s_nl) * dzy + cxs;
+            float ly = 2.0f * (fabsf(Zx) * ddy + fabsf(Zy) * ddx + ddx * ddy * s_nl) - cys;
+            dzx = lx;
+            dzy = ly;
+        } else {
+            float pr = dzx, pi = dzy, qr = Zx, qi = Zy;
+            for (int e = 0; e < v.gen_n - 1; ++e) {
+                float t1r = pr * Zx - pi * Zy, t1i = pr * Zy + pi * Zx;
+                float t2r = qr * dzx - qi * dzy, t2i = qr * dzy + qi * dzx;
+                float t3r = pr * dzx - pi * dzy, t3i = pr * dzy + pi * dzx;
+                pr = t1r + t2r + t3r * s_nl;
+                pi = t1i + t2i + t3i * s_nl;
+                float nq = qr * Zx - qi * Zy;
+                qi = qr * Zy + qi * Zx;
+                qr = nq;
+            }
+            float ndx = pr + (kr * dzx - ki * dzy) + cxs;
+            dzy = pi + (kr * dzy + ki * dzx) + cys;
+            dzx = ndx;
+        }
+        renorm32(dzx, dzy, ze);
+        ++k;
+    }
+    return fexp32_tail(v, o, ldexp((double)dzx, ze - E), ldexp((double)dzy, ze - E), dcx_d, dcy_d,
+                       k, glitch);
+}
+
 // ------------------------------------------------ colour --------------------
 double pymod(double a, double b) {
     double m = fmod(a, b);
@@ -790,6 +991,44 @@ int or_render_fexp64(const ap_view* v, const double* zr, const double* zi, const
     Orbit ob{zr, zi, V.ref_len, bla, bla ? levels : 0};
     return render(V, *o, true, threads,
                   [&](double si, double sj) { return fexp64_sample(V, ob, si, sj); }, iters);
+}
+
+// FP32-mantissa FloatExp (this build's design) over the same orbit and a
+// reference-layout table.
+int or_render_fexp32(const ap_view* v, const double* zr, const double* zi, const double* bla,
+                     int levels, const ap_outputs* o, int threads, long long* iters) {
+    const ap_view V = *v;
+    Orbit ob{zr, zi, V.ref_len, bla, bla ? levels : 0};
+    Orbit32 O;
+    O.o = &ob;
+    O.zfx.resize(V.ref_len + 1);
+    O.zfy.resize(V.ref_len + 1);
+    for (int k = 0; k <= V.ref_len; ++k) {
+        O.zfx[k] = (float)zr[k];
+        O.zfy[k] = (float)zi[k];
+    }
+    if (bla) {
+        size_t n = (size_t)V.ref_len * levels;
+        O.lr.resize(n);
+        O.c32.resize(n);
+        for (int k = 0; k < V.ref_len; ++k)
+            for (int d = 0; d < levels; ++d) {
+                const double* e = bla + ((size_t)d * V.ref_len + k) * 5;
+                size_t i = (size_t)k * levels + d;
+                O.lr[i] = e[4] > 0.0 ? (float)(0.5 * log2(e[4])) : -INFINITY;
+                double am = std::max(fabs(e[0]), fabs(e[1])), bm = std::max(fabs(e[2]), fabs(e[3]));
+                Coef32 q;
+                q.ae = am > 0.0 ? ilogb(am) : 0;
+                q.be = bm > 0.0 ? ilogb(bm) : 0;
+                q.ar = (float)ldexp(e[0], -q.ae);
+                q.ai = (float)ldexp(e[1], -q.ae);
+                q.br = (float)ldexp(e[2], -q.be);
+                q.bi = (float)ldexp(e[3], -q.be);
+                O.c32[i] = q;
+            }
+    }
+    return render(V, *o, true, threads,
+                  [&](double si, double sj) { return fexp32_sample(V, O, si, sj); }, iters);
 }
 
 // _jit_build_bla_table, math_core/high_precision.py:1100-1238; out is

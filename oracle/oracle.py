@@ -40,6 +40,7 @@ def load(fast=False):
         "or_render_dd": (I, [V, O, I, LL]),
         "or_render_pert64": (I, [V, P, P, P, I, O, I, LL]),
         "or_render_fexp64": (I, [V, P, P, P, I, O, I, LL]),
+        "or_render_fexp32": (I, [V, P, P, P, I, O, I, LL]),
         "or_bla_build": (I, [P, P, I, I, D, I, I, D, D, P]),
         "or_edge_mask": (I, [P, I, I, C.c_float, P]),
         "or_hist_build": (I, [P, C.c_longlong, I, I, P]),
@@ -94,11 +95,12 @@ def render(engine, view, lut=None, mask=None, cdf=None, orbit=None, bla=None, le
         lib.or_render_fp64(C.byref(view), C.byref(o), threads, C.byref(it))
     elif engine == "dd":
         lib.or_render_dd(C.byref(view), C.byref(o), threads, C.byref(it))
-    elif engine in ("pert64", "fexp64"):
+    elif engine in ("pert64", "fexp64", "fexp32"):
         re, im = orbit
         re = np.ascontiguousarray(re, np.float64)
         im = np.ascontiguousarray(im, np.float64)
-        fn = lib.or_render_pert64 if engine == "pert64" else lib.or_render_fexp64
+        fn = {"pert64": lib.or_render_pert64, "fexp64": lib.or_render_fexp64,
+              "fexp32": lib.or_render_fexp32}[engine]
         fn(C.byref(view), _p(re), _p(im), _p(bla), levels if bla is not None else 0, C.byref(o),
            threads, C.byref(it))
     else:

@@ -1,0 +1,82 @@
+"""The C-ABI library loads and exports every entry point the header declares
+(no compute call: this runs without a GPU)."""
+import ctypes as C
+import os
+import re
+
+from fractura_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "apeiron_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ap_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_path_entry_points():
+    syms = declared_symbols()
+    for s in ("ap_render_fp64", "ap_render_dd", "ap_render_pert64", "ap_render_fexp64",
+              "ap_render_fexp32", "ap_bla_build", "ap_edge_mask", "ap_compact", "ap_hist_build",
+              "ap_hist_cdf", "ap_colorize"):
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _abi.load()
+    for s in declared_symbols():
+        assert hasattr(lib, s), s
+        assert s in _abi.SIGNATURES, f"{s} has no ctypes signature"
+
+
+def test_struct_sizes_match_header_layout():
+    # ap_view: 8 int32, 16 doubles ... as laid out in the header.
+    assert C.sizeof(_abi.ApView) == 8 * 4 + 20 * 8 + 6 * 4 + 3 * 8
+    assert C.sizeof(_abi.ApOutputs) == 8 * 8 + 2 * 4 + 3 * 8
+    assert C.sizeof(_abi.ApOrbit) == 2 * 8 + 2 * 4
+
+
+def test_bla_bytes_is_pure_host_arithmetic():
+    lib = _abi.load()
+    assert lib.ap_bla_bytes(0, 16) == 0
+    n = lib.ap_bla_bytes(200000, 16)
+    # r2 + r1 + coef + lr + c32 per entry = 8 + 8 + 32 + 4 + 24 bytes
+    assert n >= 200000 * 16 * 76 and n < 200000 * 16 * 76 + 5 * 256
+
+
+def test_errors_are_status_codes_without_a_device():
+    lib = _abi.load()
+    h = C.c_void_p()
+    st = lib.ap_init(0, C.byref(h))
+    assert st in (_abi.AP_OK, _abi.AP_ERR_HIP)
+    if st == _abi.AP_OK:
+        lib.ap_destroy(h)
+    assert lib.ap_render_fp64(None, None, None) == _abi.AP_ERR_ARG
+
+
+def test_ctypes_layout_matches_the_c_compiler(tmp_path):
+    """offsetof/sizeof of every field, from gcc on the header, against ctypes."""
+    import shutil
+    import subprocess
+
+    import pytest
+
+    if shutil.which("gcc") is None:
+        pytest.skip("no gcc")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "apeiron_hip.h"', "int main(void){"]
+    for st, cls in (("ap_view", _abi.ApView), ("ap_outputs", _abi.ApOutputs), ("ap_orbit", _abi.ApOrbit)):
+        lines.append(f'printf("{st} size %zu\\n", sizeof({st}));')
+        for name, _ in cls._fields_:
+            lines.append(f'printf("{st} {name} %zu\\n", offsetof({st}, {name}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for st, cls in (("ap_view", _abi.ApView), ("ap_outputs", _abi.ApOutputs), ("ap_orbit", _abi.ApOrbit)):
+        assert got[(st, "size")] == C.sizeof(cls)
+        for name, _ in cls._fields_:
+            assert got[(st, name)] == getattr(cls, name).offset, (st, name)

@@ -1,0 +1,254 @@
+"""GPU parity of the perturbation engines, the device BLA build, queues and
+the rebase pass against the CPU oracle (through the C ABI)."""
+import ctypes as C
+from decimal import Decimal
+
+import numpy as np
+import pytest
+
+from fractura_amd import _abi, engine
+from tests.helpers import lut, oracle_frame, to_np
+
+pytestmark = pytest.mark.gpu
+
+SEAHORSE = (Decimal("-0.743643887037158704752191506114774"),
+            Decimal("0.131825904205311970493132056385139"))
+
+
+def _sync():
+    import torch
+
+    torch.cuda.synchronize()
+
+
+def _plan(cx, cy, w, W=96, H=54, max_iter=4000, **kw):
+    kw.setdefault("bbsa_order", 32)
+    kw.setdefault("bbsa_tol", 1e-6)
+    kw.setdefault("glitch_mode", "Pauldelbrot")
+    return engine.plan_view(W, H, max_iter, Decimal(cx), Decimal(cy), Decimal(w), **kw)
+
+
+def _oracle(plan, **kw):
+    from oracle import oracle
+
+    re, im, n = plan.orbit
+    bla = oracle.bla_build(re, im, n, plan.view.fractal_type, plan.bla_tol, 16, plan.view.gen_n,
+                           plan.view.gen_kr, plan.view.gen_ki) if plan.bla else None
+    return oracle_frame(plan, lut(), orbit=(re, im), bla=bla, levels=16 if plan.bla else 0, **kw)
+
+
+def _render(plan, **kw):
+    fr = engine.render_view(plan, "inferno", rebase=False, **kw)
+    _sync()
+    return fr
+
+
+def _smooth_close(a, b, tol):
+    esc = (a >= 0) & (b >= 0)
+    return np.max(np.abs(a[esc] - b[esc]), initial=0.0) <= tol
+
+
+@pytest.mark.parametrize("ftype,cx,cy,n", [(0, *SEAHORSE, 3), (0, "0", "1", 3), (4, "0.1", "0.8", 3),
+                                           (2, "-0.1", "0.2", 3), (4, "0.3", "0.5", 5)])
+def test_bla_build_bit_exact(ftype, cx, cy, n):
+    """Device level-by-level merge == the CPU restatement, bit for bit."""
+    import torch
+
+    from oracle import oracle
+
+    plan = _plan(cx, cy, "1e-14", max_iter=3000, fractal_type=ftype, gen_n=n, julia_cx=-0.67,
+                 julia_cy=0.37)
+    re, im, ref_len = plan.orbit
+    dev = engine.get_device(0)
+    dev.set_stream(None)
+    orbit = engine._Orbit(dev, plan, 0)
+    out = torch.empty((16, ref_len, 5), dtype=torch.float64, device=dev.torch_device)
+    dev.call("ap_bla_export", engine.ptr(orbit.table), ref_len, 16, engine.ptr(out))
+    _sync()
+    ref = oracle.bla_build(re, im, ref_len, ftype, plan.bla_tol, 16, n, 0.25, 1.0)
+    got = to_np(out)
+    if n <= 4:
+        assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+    else:
+        # n > 4 takes pow() for |Z|^(n-2) and tol_a^(1/(n-1)); the device and
+        # libm pow may differ in the last bit, so the radii (and only they) get
+        # a few ulps; the level-0 coefficients stay exact.
+        assert np.array_equal(got[0, :, :4].view(np.uint64), ref[0, :, :4].view(np.uint64))
+        assert np.allclose(got, ref, rtol=1e-12, atol=0)
+
+
+def test_pert64_seahorse_matches_oracle():
+    plan = _plan(*SEAHORSE, "1e-14", max_iter=5000)
+    assert plan.engine == "pert64" and plan.bla
+    fr = _render(plan)
+    m = _oracle(plan)
+    c = to_np(fr.counts)
+    assert np.array_equal(c, m.counts), f"{np.sum(c != m.counts)} of {c.size} counts differ"
+    assert np.array_equal(to_np(fr.glitch), m.glitch)
+    assert _smooth_close(to_np(fr.smooth), m.smooth, 1e-9)
+    assert fr.host_stats()["iterations"] == m.iterations
+
+
+@pytest.mark.parametrize("ftype", [0, 1, 2, 3, 4])
+def test_pert64_types_no_bla(ftype):
+    cx, cy = {0: SEAHORSE, 1: ("-1.7620", "-0.0280"), 2: ("0.0", "0.0"), 3: ("0.0", "0.0"),
+              4: ("0.1", "0.8")}[ftype]
+    plan = _plan(cx, cy, "1e-13", max_iter=1500, fractal_type=ftype, bbsa_order=4,
+                 julia_cx=-0.67, julia_cy=0.37)
+    assert not plan.bla
+    fr = _render(plan)
+    m = _oracle(plan)
+    assert np.array_equal(to_np(fr.counts), m.counts)
+    assert np.array_equal(to_np(fr.glitch), m.glitch)
+
+
+@pytest.mark.parametrize("ftype,cx,cy", [(0, "0", "1"), (4, "0.1", "0.8"), (1, "-1.7620", "-0.0280"),
+                                         (2, "0.0", "0.0")])
+def test_fexp64_matches_oracle(ftype, cx, cy):
+    plan = _plan(cx, cy, "1e-320", max_iter=4000, fractal_type=ftype, julia_cx=-0.67, julia_cy=0.37)
+    assert plan.engine == "fexp64"
+    fr = _render(plan)
+    m = _oracle(plan)
+    c = to_np(fr.counts)
+    # The Pauldelbrot exponent comes from frexp on the device and from
+    # int(log(x)/ln 2) in the oracle; they can differ within an ulp of a
+    # power of two, which may flip a glitch flag but not a count.
+    assert np.array_equal(c, m.counts), f"{np.sum(c != m.counts)} counts differ"
+    assert np.mean(to_np(fr.glitch) == m.glitch) >= 0.999
+    assert _smooth_close(to_np(fr.smooth), m.smooth, 1e-9)
+
+
+def _err_stats(x_counts, x_smooth, t_counts, t_smooth):
+    same = x_counts == t_counts
+    d = np.abs(x_smooth - t_smooth)
+    d[~same | (t_smooth < 0)] = 0.0
+    return float(np.mean(same)), float(np.quantile(d, 0.99)), float(np.median(d[t_smooth >= 0])) if np.any(t_smooth >= 0) else 0.0
+
+
+@pytest.mark.parametrize("cx,cy,w,ftype", [("0", "1", "1e-320", 0), ("0", "1", "1e-1000", 0),
+                                           ("0.1", "0.8", "1e-320", 4),
+                                           ("-1.7620", "-0.0280", "1e-320", 1)])
+def test_fexp32_accuracy(cx, cy, w, ftype):
+    """FP32-mantissa FloatExp + BLA against exact FP64 perturbation (no BLA):
+    >= 99.9% count agreement, p99 smooth error <= 1e-4 and no worse than the
+    FP64-mantissa FloatExp + BLA design; and against its CPU restatement."""
+    from oracle import oracle
+
+    plan32 = _plan(cx, cy, w, max_iter=5000, fractal_type=ftype, fexp_mantissa=32)
+    assert plan32.engine == "fexp32"
+    fr = _render(plan32)
+    c32, s32 = to_np(fr.counts), to_np(fr.smooth)
+    re, im, n = plan32.orbit
+    truth = oracle.render("fexp64", plan32.view, orbit=(re, im))
+    bla = oracle.bla_build(re, im, n, ftype, plan32.bla_tol, 16) if plan32.bla else None
+    lv = 16 if plan32.bla else 0
+    m64 = oracle.render("fexp64", plan32.view, orbit=(re, im), bla=bla, levels=lv)
+    m32 = oracle.render("fexp32", plan32.view, orbit=(re, im), bla=bla, levels=lv)
+    agree, p99, med = _err_stats(c32, s32, truth.counts, truth.smooth)
+    agree64, p99_64, _ = _err_stats(m64.counts, m64.smooth, truth.counts, truth.smooth)
+    assert agree >= 0.999, agree
+    assert p99 <= max(1e-4, 1.5 * p99_64), (p99, p99_64)
+    assert med <= 1e-5
+    # The device kernel and its CPU restatement: the same arithmetic except
+    # the hardware log2 in the BLA validity test.
+    assert np.mean(c32 == m32.counts) >= 0.999
+    same = c32 == m32.counts
+    assert np.quantile(np.abs(s32 - m32.smooth)[same], 0.99) <= 1e-6
+
+
+def test_fexp64_at_e0_agrees_with_pert64():
+    """FloatExp with E = 0 is the FP64-delta loop in another arrangement."""
+    a = _plan("-0.65", "0.0", "3.0", W=128, H=72, max_iter=300, engine_override="fexp64",
+              bbsa_order=4)
+    b = _plan("-0.65", "0.0", "3.0", W=128, H=72, max_iter=300, engine_override="pert64",
+              bbsa_order=4)
+    assert a.view.e_scale == 0
+    ca = to_np(_render(a).counts)
+    cb = to_np(_render(b).counts)
+    assert np.mean(ca == cb) >= 0.99
+
+
+@pytest.mark.parametrize("engine_name,w", [("pert64", "1e-14"), ("fexp64", "1e-320")])
+@pytest.mark.parametrize("factor,scheme", [(2, 0), (3, 11)])
+def test_perturbation_ssaa_and_histogram(engine_name, w, factor, scheme):
+    cx, cy = SEAHORSE if engine_name == "pert64" else ("0", "1")
+    plan = _plan(cx, cy, w, W=64, H=36, max_iter=4000, ssaa_factor=factor,
+                 color_scheme_id=scheme)
+    assert plan.engine == engine_name
+    fr = _render(plan)
+    m = _oracle(plan)
+    rgb = to_np(fr.rgb).astype(int)
+    diff = np.abs(rgb - m.rgb.astype(int))
+    assert diff.max() <= 1 and np.mean(diff == 0) >= 0.99
+    assert np.array_equal(to_np(fr.glitch), m.glitch) or engine_name == "fexp64"
+    it = to_np(fr.iter_map)
+    assert np.mean(np.abs(it - m.iter_map) <= 1e-4 * np.maximum(1, np.abs(m.iter_map))) >= 0.999
+
+
+def test_compact_and_glitch_bins_match_numpy():
+    import torch
+
+    dev = engine.get_device(0)
+    dev.set_stream(None)
+    rng = np.random.default_rng(4)
+    for W, H, p in [(1, 1, 0.5), (37, 19, 0.1), (640, 360, 0.02), (1920, 1080, 0.6), (333, 77, 0.0)]:
+        m = (rng.random((H, W)) < p).astype(np.uint8)
+        md = torch.from_numpy(m).to(dev.torch_device)
+        q = torch.full((W * H,), -5, dtype=torch.int32, device=dev.torch_device)
+        n = torch.zeros(1, dtype=torch.int32, device=dev.torch_device)
+        bins = torch.zeros(256, dtype=torch.int32, device=dev.torch_device)
+        dev.call("ap_compact", engine.ptr(md), W * H, engine.ptr(q), engine.ptr(n))
+        dev.call("ap_glitch_bins", engine.ptr(md), W, H, engine.ptr(bins))
+        _sync()
+        idx = np.flatnonzero(m)
+        assert int(n.item()) == idx.size
+        assert np.array_equal(to_np(q)[: idx.size], idx)
+        r, c = idx // W, idx % W
+        b = np.clip(r * 16 // H, 0, 15) * 16 + np.clip(c * 16 // W, 0, 15)
+        assert np.array_equal(to_np(bins), np.bincount(b, minlength=256))
+
+
+def test_rebase_pass_matches_oracle():
+    """One rebase pass: the GPU re-renders the compacted glitch queue around a
+    new reference; the oracle re-renders the same pixels through the glitch
+    mask; the frames agree."""
+    from oracle import oracle
+
+    plan = _plan(*SEAHORSE, "1e-14", max_iter=5000)
+    fr = engine.render_view(plan, "inferno", glitch_mode="Pauldelbrot", rebase=True)
+    _sync()
+    assert fr.rebase_passes >= 1
+    assert fr.glitch_counts[0] > 0
+    # Replay the same passes on the oracle.
+    m = _oracle(plan)
+    v = plan.view
+    W, H = v.width, v.height
+    visited = {(int(np.clip(v.ref_py, 0, H - 1)), int(np.clip(v.ref_px, 0, W - 1)))}
+    from fractura_amd.high_precision import compute_single_highprec_orbit
+    from fractura_amd.config import compute_required_prec_bits, decimal_context
+    for p in range(fr.rebase_passes):
+        idx = np.flatnonzero(m.glitch)
+        assert idx.size == fr.glitch_counts[p]
+        pick = engine._select_rebase_pixel(idx, W, H, m.iter_map.ravel()[idx], v.max_iter, visited)
+        visited.add(pick)
+        g_row, g_col = pick
+        with decimal_context(plan.plot_width_dec):
+            x_min = plan.center_x_dec - plan.plot_width_dec / Decimal(2)
+            y_max = plan.center_y_dec + plan.plot_height_dec / Decimal(2)
+            rcx = x_min + (Decimal(g_col) / Decimal(W - 1)) * plan.plot_width_dec
+            rcy = y_max - (Decimal(g_row) / Decimal(H - 1)) * plan.plot_height_dec
+            re, im, rl = compute_single_highprec_orbit(
+                rcx, rcy, v.max_iter, 0, 0.0, 0.0, compute_required_prec_bits(plan.plot_width_dec))
+        rv = v.copy()
+        rv.rebase_pass = p + 1
+        rv.mode = 0
+        rv.ref_px, rv.ref_py = float(g_col), float(g_row)
+        rv.ref_cx, rv.ref_cy, rv.ref_len = float(rcx), float(rcy), rl
+        bla = oracle.bla_build(re, im, rl, 0, plan.bla_tol, 16)
+        gin = m.glitch.copy()
+        oracle.render("pert64", rv, lut=lut(), mask=gin, orbit=(re[: rl + 1], im[: rl + 1]),
+                      bla=bla, levels=16, maps=m)
+    c = to_np(fr.counts)
+    assert np.array_equal(c, m.counts), f"{np.sum(c != m.counts)} counts differ"
+    assert np.array_equal(to_np(fr.glitch), m.glitch)
+    assert int(m.glitch.sum()) < fr.glitch_counts[0]
