@@ -79,6 +79,8 @@ int check_view(ap_ctx* c, const ap_view* v, const ap_outputs* o) {
         return fail(c, AP_ERR_ARG, "rebase pass needs a glitch mask or a queue");
     if (o->queue && !o->queue_count) return fail(c, AP_ERR_ARG, "queue without a device count");
     if (o->rgb && !o->lut) return fail(c, AP_ERR_ARG, "rgb output needs a LUT");
+    if (o->rgb && v->mode == AP_MODE_CENTRE && !o->iter_map)
+        return fail(c, AP_ERR_ARG, "a centre pass colours its float32 map: rgb needs iter_map");
     if (o->cdf && o->num_bins < 2) return fail(c, AP_ERR_ARG, "cdf needs num_bins >= 2");
     if (v->scheme_id < 0 || v->scheme_id > 11) return fail(c, AP_ERR_ARG, "bad scheme %d", v->scheme_id);
     return AP_OK;

@@ -19,57 +19,88 @@ constexpr double kBailout2 = 65536.0;  // |z|^2 > 65536, R = 256 (math_core/cpu_
 constexpr int kLutSize = 2048;
 
 // ---------------------------------------------------------------- results --
+// Engines return the escape index and |z|^2 at escape; the smooth values are
+// computed once per sample by finish(), so the transcendental epilogue is
+// not replicated at every exit of the iteration loops.
+enum : int { kEscaped = 0, kInterior = 1, kNanExit = 2 };
+
 struct Sample {
-    int count;      // escape index, max_iter if never escaped, -1 for shortcut
-    float s;        // reference float32 smooth value (-1 for the interior)
-    double smooth;  // float64 smooth value (-1 for the interior)
-    int glitch;     // perturbation glitch flag
-    long long iters;  // iterations advanced (BLA leaps count their length)
-    int plain;      // executed single-iteration steps
-    int bla;        // executed BLA leaps
+    int count;        // escape index, max_iter if never escaped, -1 for shortcut
+    int state;        // kEscaped / kInterior / kNanExit
+    double mag2;      // |z|^2 at escape
+    int glitch;       // perturbation glitch flag
+    int plain;        // executed single-iteration steps
+    int bla;          // executed BLA leaps
+    float s;          // reference float32 smooth value (-1 for the interior)
+    double smooth;    // float64 smooth value (-1 for the interior)
 };
 
-__device__ inline Sample interior(int count, long long iters) {
+__device__ inline Sample interior(int count, long long /*iters*/) {
     Sample r;
     r.count = count;
-    r.s = -1.0f;
-    r.smooth = -1.0;
+    r.state = kInterior;
+    r.mag2 = 0.0;
     r.glitch = 0;
-    r.iters = iters;
     r.plain = 0;
     r.bla = 0;
     return r;
 }
+
+__device__ inline Sample escaped(int k, double mag2, double, double, double) {
+    Sample r;
+    r.count = k;
+    r.state = kEscaped;
+    r.mag2 = mag2;
+    r.glitch = 0;
+    r.plain = 0;
+    r.bla = 0;
+    return r;
+}
+
+// Iterations a sample advanced (BLA leaps count their length; the cardioid
+// shortcut counts 0).
+__device__ inline long long sample_iters(const Sample& r) { return r.count > 0 ? r.count : 0; }
 
 // ln(ln 256) = ln(0.5 ln 65536), correctly rounded.  The reference passes
 // 1.7129177284488854 (config.py:26), 1.09e-5 below it; the float32 map keeps
 // the reference's constant, the float64 smooth value uses this one.
 constexpr double kLnLn256 = 1.7129286210981716;
 
-// Escape epilogue of every reference sample evaluator
+// The escape epilogue of every reference sample evaluator
 // (math_core/cpu_kernels.py:246-249): the float32 value is
 // f32(k+1) - f32((ln(0.5 ln|z|^2) - LN_BAILOUT_LOG) * inv_ln) with float32
 // subtraction; the float64 value is k+1 - log2(ln|z| / ln 256) in double, with
 // the exact ln(ln 256) and the exact 1/ln n for type 4 (the reference rounds
-// 1/ln n to float32).
-__device__ inline Sample escaped(int k, double mag2, double inv_ln_ref,
-                                 double inv_ln_exact, double ln_bailout_log) {
-    Sample r;
-    r.count = k;
-    r.glitch = 0;
-    r.iters = k;
-    r.plain = 0;
-    r.bla = 0;
-    double log_mag = log(mag2);
-    if (log_mag > 1.0) {
-        double ll = log(0.5 * log_mag);
-        r.s = (float)(k + 1.0) - (float)((ll - ln_bailout_log) * inv_ln_ref);
-        r.smooth = (double)(k + 1) - (ll - kLnLn256) * inv_ln_exact;
+// 1/ln n to float32).  A NaN/Inf exit reports s = k.
+__device__ inline void finish(Sample& r, double inv_ln_ref, double inv_ln_exact,
+                              double ln_bailout_log) {
+    if (r.state == kEscaped) {
+        const int k = r.count;
+        double log_mag = log(r.mag2);
+        if (log_mag > 1.0) {
+            double ll = log(0.5 * log_mag);
+            r.s = (float)(k + 1.0) - (float)((ll - ln_bailout_log) * inv_ln_ref);
+            r.smooth = (double)(k + 1) - (ll - kLnLn256) * inv_ln_exact;
+        } else {
+            r.s = (float)k;
+            r.smooth = (double)k;
+        }
+    } else if (r.state == kNanExit) {
+        r.s = (float)r.count;
+        r.smooth = (double)r.count;
     } else {
-        r.s = (float)k;
-        r.smooth = (double)k;
+        r.s = -1.0f;
+        r.smooth = -1.0;
     }
-    return r;
+}
+
+// 1/ln 2 for types 0..3; for type 4 the reference uses f32(1/ln n).
+__device__ inline void inv_ln_constants(const ap_view& v, double& ref, double& exact) {
+    ref = exact = v.inv_ln2;
+    if (v.fractal_type == AP_GENERAL_MANDELBROT && v.gen_n > 1) {
+        exact = 1.0 / log((double)v.gen_n);
+        ref = (double)(float)exact;
+    }
 }
 
 // Type 0 cardioid / period-2 bulb shortcut (math_core/cpu_kernels.py:229-234).

@@ -12,20 +12,25 @@
 #pragma once
 
 #include "ap_device.hpp"
+#include "ap_launch.hpp"
 
 namespace ap {
 
 struct LaneStats {
     unsigned long long iters = 0, plain = 0, bla = 0, samples = 0;
     __device__ void add(const Sample& r) {
-        iters += (unsigned long long)(r.iters > 0 ? r.iters : 0);
+        iters += (unsigned long long)sample_iters(r);
         plain += (unsigned long long)r.plain;
         bla += (unsigned long long)r.bla;
         samples += 1;
     }
 };
 
-template <class Eng>
+// One pixel.  kColour: the factor^2-subsample passes (modes 1 and 2) colour
+// every subsample here to average RGB; the centre pass (mode 0) writes the
+// maps only and RGB comes from the colourise pass, so the palette code does
+// not weigh on the registers of the iteration loop.
+template <class Eng, bool kColour>
 __device__ inline void process_pixel(const ap_view& v, const ap_outputs& o,
                                      const typename Eng::Data& d, int x, int y,
                                      bool from_queue, LaneStats& st) {
@@ -34,64 +39,64 @@ __device__ inline void process_pixel(const ap_view& v, const ap_outputs& o,
         if (v.rebase_pass > 0 && o.mask && o.mask[idx] == 0) return;
         if (v.mode == AP_MODE_EDGE_SUBSAMPLES && v.rebase_pass == 0 && o.mask[idx] == 0) return;
     }
+    double inv_ref, inv_exact;
+    inv_ln_constants(v, inv_ref, inv_exact);
     const int f = v.factor;
-    if (v.mode == AP_MODE_CENTRE) {
+    if (!kColour) {
         const double off = ((double)f - 1.0) * 0.5;
         Sample r = Eng::sample(v, d, (double)(x * f) + off, (double)(y * f) + off);
+        finish(r, inv_ref, inv_exact, v.ln_bailout_log);
         st.add(r);
         if (o.counts) o.counts[idx] = r.count;
         if (o.smooth) o.smooth[idx] = r.smooth;
         if (o.iter_map) o.iter_map[idx] = r.s;
         if (o.glitch) o.glitch[idx] = r.glitch ? 1 : 0;
-        if (o.rgb) {
-            double cr = 0.0, cg = 0.0, cb = 0.0;
-            if (r.s >= 0.0f) sample_colour(v, o, r.s, cr, cg, cb);
-            store_rgb(o.rgb, v.is_hdr, idx, cr, cg, cb);
-        }
         return;
-    }
-    double ar = 0.0, ag = 0.0, ab = 0.0;
-    int glitches = 0;
-    Sample r;
-    for (int sy = 0; sy < f; ++sy) {
-        for (int sx = 0; sx < f; ++sx) {
-            r = Eng::sample(v, d, (double)(x * f + sx), (double)(y * f + sy));
-            st.add(r);
-            if (r.glitch) ++glitches;
-            if (r.s >= 0.0f) {
-                double cr, cg, cb;
-                sample_colour(v, o, r.s, cr, cg, cb);
-                ar += cr;
-                ag += cg;
-                ab += cb;
+    } else {
+        double ar = 0.0, ag = 0.0, ab = 0.0;
+        int glitches = 0;
+        Sample r;
+        for (int sy = 0; sy < f; ++sy) {
+            for (int sx = 0; sx < f; ++sx) {
+                r = Eng::sample(v, d, (double)(x * f + sx), (double)(y * f + sy));
+                finish(r, inv_ref, inv_exact, v.ln_bailout_log);
+                st.add(r);
+                if (r.glitch) ++glitches;
+                if (r.s >= 0.0f) {
+                    double cr, cg, cb;
+                    sample_colour(v, o, r.s, cr, cg, cb);
+                    ar += cr;
+                    ag += cg;
+                    ab += cb;
+                }
             }
         }
-    }
-    const double inv = 1.0 / (double)(f * f);
-    if (o.rgb) store_rgb(o.rgb, v.is_hdr, idx, ar * inv, ag * inv, ab * inv);
-    if (o.glitch) o.glitch[idx] = glitches > 0 ? 1 : 0;
-    // The perturbation drivers leave the last subsample in the float32 map
-    // (math_core/cuda_kernels.py:1824,1917); the direct drivers and the
-    // histogram resolve (cdf given) do not, except in rebase passes (:2173).
-    if ((Eng::kIterInSubsamples && !o.cdf) || v.rebase_pass > 0) {
-        if (o.iter_map) o.iter_map[idx] = r.s;
-        if (o.counts) o.counts[idx] = r.count;
-        if (o.smooth) o.smooth[idx] = r.smooth;
+        const double inv = 1.0 / (double)(f * f);
+        if (o.rgb) store_rgb(o.rgb, v.is_hdr, idx, ar * inv, ag * inv, ab * inv);
+        if (o.glitch) o.glitch[idx] = glitches > 0 ? 1 : 0;
+        // The perturbation drivers leave the last subsample in the float32 map
+        // (math_core/cuda_kernels.py:1824,1917); the direct drivers and the
+        // histogram resolve (cdf given) do not, except in rebase passes (:2173).
+        if ((Eng::kIterInSubsamples && !o.cdf) || v.rebase_pass > 0) {
+            if (o.iter_map) o.iter_map[idx] = r.s;
+            if (o.counts) o.counts[idx] = r.count;
+            if (o.smooth) o.smooth[idx] = r.smooth;
+        }
     }
 }
 
-template <class Eng>
+template <class Eng, bool kColour>
 __global__ __launch_bounds__(kBlock) void tile_pass(ap_view v, ap_outputs o,
                                                    typename Eng::Data d) {
     LaneStats st;
     int x, y;
-    if (tile_pixel(v.width, v.height, x, y)) process_pixel<Eng>(v, o, d, x, y, false, st);
+    if (tile_pixel(v.width, v.height, x, y)) process_pixel<Eng, kColour>(v, o, d, x, y, false, st);
     flush_stats(o.stats, st.iters, st.plain, st.bla, st.samples);
 }
 
 // Compacted queue: a grid-stride loop over the device-side count, so the
 // host needs no copy-back of the count before launching.
-template <class Eng>
+template <class Eng, bool kColour>
 __global__ __launch_bounds__(kBlock) void queue_pass(ap_view v, ap_outputs o,
                                                     typename Eng::Data d) {
     LaneStats st;
@@ -102,7 +107,7 @@ __global__ __launch_bounds__(kBlock) void queue_pass(ap_view v, ap_outputs o,
         int q = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
         if (q < n) {
             int p = o.queue[q];
-            process_pixel<Eng>(v, o, d, p % v.width, p / v.width, true, st);
+            process_pixel<Eng, kColour>(v, o, d, p % v.width, p / v.width, true, st);
         }
     }
     flush_stats(o.stats, st.iters, st.plain, st.bla, st.samples);
@@ -111,14 +116,24 @@ __global__ __launch_bounds__(kBlock) void queue_pass(ap_view v, ap_outputs o,
 template <class Eng>
 inline hipError_t launch_pass(const ap_view& v, const ap_outputs& o,
                               const typename Eng::Data& d, hipStream_t s) {
+    const bool subsamples = v.mode != AP_MODE_CENTRE;
     if (o.queue) {
         int blocks = 1024;  // 4 blocks per CU; work per lane varies anyway
-        hipLaunchKernelGGL(queue_pass<Eng>, dim3(blocks), dim3(kBlock), 0, s, v, o, d);
+        if (subsamples)
+            hipLaunchKernelGGL((queue_pass<Eng, true>), dim3(blocks), dim3(kBlock), 0, s, v, o, d);
+        else
+            hipLaunchKernelGGL((queue_pass<Eng, false>), dim3(blocks), dim3(kBlock), 0, s, v, o, d);
     } else {
         dim3 grid((v.width + kTile - 1) / kTile, (v.height + kTile - 1) / kTile);
-        hipLaunchKernelGGL(tile_pass<Eng>, grid, dim3(kBlock), 0, s, v, o, d);
+        if (subsamples)
+            hipLaunchKernelGGL((tile_pass<Eng, true>), grid, dim3(kBlock), 0, s, v, o, d);
+        else
+            hipLaunchKernelGGL((tile_pass<Eng, false>), grid, dim3(kBlock), 0, s, v, o, d);
     }
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || subsamples || !o.rgb) return e;
+    // Centre pass with RGB requested: colour the map it just wrote.
+    return launch_colorize(v, o.iter_map, o.cdf, o.num_bins, o.lut, o.rgb, s);
 }
 
 }  // namespace ap

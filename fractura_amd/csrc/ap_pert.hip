@@ -81,10 +81,9 @@ __device__ inline void renorm64(double& dzx, double& dzy, int& z_exp) {
 __device__ inline Sample nan_exit(int k) {
     Sample r;
     r.count = k;
-    r.s = (float)k;
-    r.smooth = (double)k;
+    r.state = kNanExit;
+    r.mag2 = 0.0;
     r.glitch = 1;
-    r.iters = k;
     r.plain = 0;
     r.bla = 0;
     return r;

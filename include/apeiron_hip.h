@@ -107,7 +107,9 @@ typedef struct ap_outputs {
                              -1 for the interior                             */
     float *iter_map;      /* reference float32 smooth map (-1 interior)      */
     uint8_t *glitch;      /* glitch flags (perturbation engines)             */
-    void *rgb;            /* (H, W, 3) uint8, or uint16 if view.is_hdr       */
+    void *rgb;            /* (H, W, 3) uint8, or uint16 if view.is_hdr; a
+                             centre pass (mode 0) colours iter_map after the
+                             iteration kernel, so it needs iter_map too       */
     const float *lut;     /* (2048, 3) float32 colour LUT                    */
     const uint8_t *mask;  /* edge mask (mode 2) or glitch-in mask (rebase)   */
     const float *cdf;     /* histogram CDF: colour subsamples by scheme 11   */
