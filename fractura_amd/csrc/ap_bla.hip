@@ -12,11 +12,12 @@
 namespace ap {
 
 struct BlaMut {
-    double* r2;
-    double* r1;
-    double4* coef;
-    float* lr;
-    Coef32* c32;
+    double4* coef;  // [levels][ref_len]
+    double* r2m;    // [levels][ref_len]
+    double* r2;     // [ref_len][levels]
+    double* r1;     // [ref_len][levels]
+    float* lr;      // [ref_len][levels]
+    Coef32* c32;    // [levels][ref_len]
     int levels;
     int ref_len;
 };
@@ -25,9 +26,10 @@ static BlaMut bla_mut(void* table, int ref_len, int levels) {
     BlaLayout L = bla_layout(ref_len, levels);
     char* p = (char*)table;
     BlaMut b;
+    b.coef = (double4*)(p + L.coef);
+    b.r2m = (double*)(p + L.r2m);
     b.r2 = (double*)(p + L.r2);
     b.r1 = (double*)(p + L.r1);
-    b.coef = (double4*)(p + L.coef);
     b.lr = (float*)(p + L.lr);
     b.c32 = (Coef32*)(p + L.c32);
     b.levels = levels;
@@ -40,10 +42,9 @@ __global__ __launch_bounds__(256) void bla_level0_kernel(BlaMut t, const double*
                                                          int n, double kr, double ki, double tol) {
     int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= t.ref_len) return;
-    size_t e = (size_t)k * t.levels;
     if (k >= t.ref_len - 1) {
-        t.coef[e] = make_double4(0.0, 0.0, 0.0, 0.0);
-        t.r2[e] = 0.0;
+        t.coef[k] = make_double4(0.0, 0.0, 0.0, 0.0);
+        t.r2m[k] = 0.0;
         return;
     }
     double rx = zr[k], ry = zi[k];
@@ -84,22 +85,23 @@ __global__ __launch_bounds__(256) void bla_level0_kernel(BlaMut t, const double*
         if (r_sq > 0.25) r_sq = 0.25;
         if (k == 0 && r_mag2 < 1e-14) r_sq = 0.0;
     }
-    t.coef[e] = make_double4(ar, ai, 1.0, 0.0);
-    t.r2[e] = r_sq;
+    t.coef[k] = make_double4(ar, ai, 1.0, 0.0);
+    t.r2m[k] = r_sq;
 }
 
+// Level d from level d-1, both level-major: lanes k and k+1 read adjacent
+// words, so every load and store is coalesced.
 __global__ __launch_bounds__(256) void bla_merge_kernel(BlaMut t, int d) {
     int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= t.ref_len) return;
-    const int L = t.levels;
-    size_t e = (size_t)k * L + d;
+    const size_t N = (size_t)t.ref_len;
     const int step = 1 << (d - 1);
     double4 out = make_double4(0.0, 0.0, 0.0, 0.0);
     double r_out = 0.0;
     if (k < t.ref_len - 2 * step) {
-        size_t e1 = (size_t)k * L + (d - 1);
-        size_t e2 = (size_t)(k + step) * L + (d - 1);
-        double r1_sq = t.r2[e1], r2_sq = t.r2[e2];
+        size_t e1 = (size_t)(d - 1) * N + k;
+        size_t e2 = e1 + step;
+        double r1_sq = t.r2m[e1], r2_sq = t.r2m[e2];
         if (r1_sq > 0.0 && r2_sq > 0.0) {
             double4 c1 = t.coef[e1], c2 = t.coef[e2];
             double ar = c2.x * c1.x - c2.y * c1.y;
@@ -127,43 +129,68 @@ __global__ __launch_bounds__(256) void bla_merge_kernel(BlaMut t, int d) {
             }
         }
     }
+    size_t e = (size_t)d * N + k;
     t.coef[e] = out;
-    t.r2[e] = r_out;
+    t.r2m[e] = r_out;
 }
 
-// Derived regions: sqrt(r^2) for the FP64 engines' tiny-delta test, and the
-// float32 forms for the FP32-mantissa engine.
-__global__ __launch_bounds__(256) void bla_derive_kernel(BlaMut t, size_t n) {
-    size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= n) return;
-    double r2 = t.r2[e];
-    t.r1[e] = sqrt(r2);
-    t.lr[e] = r2 > 0.0 ? (float)(0.5 * log2(r2)) : -INFINITY;
-    double4 c = t.coef[e];
-    Coef32 q;
-    double am = fmax(fabs(c.x), fabs(c.y));
-    double bm = fmax(fabs(c.z), fabs(c.w));
-    q.ae = am > 0.0 ? ilogb(am) : 0;
-    q.be = bm > 0.0 ? ilogb(bm) : 0;
-    q.ar = (float)ldexp(c.x, -q.ae);
-    q.ai = (float)ldexp(c.y, -q.ae);
-    q.br = (float)ldexp(c.z, -q.be);
-    q.bi = (float)ldexp(c.w, -q.be);
-    t.c32[e] = q;
+// Derived regions.  A block takes 16 consecutive k for every level: reads the
+// level-major radii coalesced, transposes them through LDS and writes the
+// per-k rows (r^2, sqrt(r^2) for the FP64 engines' tiny-delta test, log2 r for
+// the FP32 engine); the FP32 coefficient forms stay level-major.
+constexpr int kDeriveK = 16;
+
+__global__ __launch_bounds__(256) void bla_derive_kernel(BlaMut t) {
+    __shared__ double tile[32][kDeriveK + 1];
+    const int L = t.levels;
+    const size_t N = (size_t)t.ref_len;
+    const int k0 = blockIdx.x * kDeriveK;
+    for (int i = threadIdx.x; i < L * kDeriveK; i += 256) {
+        int d = i / kDeriveK, kl = i % kDeriveK;
+        int k = k0 + kl;
+        if (k < t.ref_len) {
+            size_t e = (size_t)d * N + k;
+            tile[d][kl] = t.r2m[e];
+            double4 c = t.coef[e];
+            Coef32 q;
+            double am = fmax(fabs(c.x), fabs(c.y));
+            double bm = fmax(fabs(c.z), fabs(c.w));
+            q.ae = am > 0.0 ? ilogb(am) : 0;
+            q.be = bm > 0.0 ? ilogb(bm) : 0;
+            q.ar = (float)ldexp(c.x, -q.ae);
+            q.ai = (float)ldexp(c.y, -q.ae);
+            q.br = (float)ldexp(c.z, -q.be);
+            q.bi = (float)ldexp(c.w, -q.be);
+            t.c32[e] = q;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < L * kDeriveK; i += 256) {
+        int kl = i / L, d = i % L;
+        int k = k0 + kl;
+        if (k < t.ref_len) {
+            double r2 = tile[d][kl];
+            size_t row = (size_t)k * L + d;
+            t.r2[row] = r2;
+            t.r1[row] = sqrt(r2);
+            t.lr[row] = r2 > 0.0 ? (float)(0.5 * log2(r2)) : -INFINITY;
+        }
+    }
 }
 
-__global__ __launch_bounds__(256) void bla_export_kernel(BlaView t, double* __restrict__ out) {
-    size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(256) void bla_export_kernel(BlaView t, const double* __restrict__ r2m,
+                                                         double* __restrict__ out) {
+    size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;  // level-major index
     size_t n = (size_t)t.ref_len * t.levels;
     if (e >= n) return;
-    int k = (int)(e / t.levels), d = (int)(e % t.levels);
+    int d = (int)(e / t.ref_len), k = (int)(e % t.ref_len);
     double4 c = t.coef[e];
     double* o = out + ((size_t)d * t.ref_len + k) * 5;
     o[0] = c.x;
     o[1] = c.y;
     o[2] = c.z;
     o[3] = c.w;
-    o[4] = t.r2[e];
+    o[4] = r2m[e];
 }
 
 size_t bla_bytes(int ref_len, int levels) { return bla_layout(ref_len, levels).total; }
@@ -177,27 +204,19 @@ hipError_t launch_bla_build(const ap_orbit& orb, int type, int n, double kr, dou
                        n, kr, ki, tol);
     for (int d = 1; d < levels; ++d)
         hipLaunchKernelGGL(bla_merge_kernel, dim3(blocks), dim3(256), 0, s, t, d);
-    size_t ne = bla_entries(ref_len, levels);
-    hipLaunchKernelGGL(bla_derive_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, t,
-                       ne);
+    hipLaunchKernelGGL(bla_derive_kernel, dim3((ref_len + kDeriveK - 1) / kDeriveK), dim3(256), 0,
+                       s, t);
     return hipGetLastError();
 }
 
 hipError_t launch_bla_export(const void* table, int ref_len, int levels, double* out,
                              hipStream_t s) {
+    BlaView v = bla_view(table, ref_len, levels);
     BlaLayout L = bla_layout(ref_len, levels);
-    const char* p = (const char*)table;
-    BlaView v;
-    v.r2 = (const double*)(p + L.r2);
-    v.r1 = (const double*)(p + L.r1);
-    v.coef = (const double4*)(p + L.coef);
-    v.lr = (const float*)(p + L.lr);
-    v.c32 = (const Coef32*)(p + L.c32);
-    v.levels = levels;
-    v.ref_len = ref_len;
+    const double* r2m = (const double*)((const char*)table + L.r2m);
     size_t ne = bla_entries(ref_len, levels);
     hipLaunchKernelGGL(bla_export_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, v,
-                       out);
+                       r2m, out);
     return hipGetLastError();
 }
 

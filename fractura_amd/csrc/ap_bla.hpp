@@ -3,30 +3,35 @@
 //
 // Entry (k, d) advances the perturbation delta from iteration k by 2^d
 // iterations: dz' = A dz + B dc, valid while |dz| < r (reference
-// math_core/high_precision.py:1100-1238).  The reference stores
-// float64[levels][ref_len][5]; this table stores, for each k, the levels of
-// one quantity contiguously, so the level search of one lane reads one
-// 128-byte row instead of `levels` scattered words:
-//   r2   double [ref_len][levels]      r^2 (0 = invalid)
-//   r1   double [ref_len][levels]      sqrt(r^2), the value the reference
-//                                      computes at lookup when |dz| is tiny
-//   coef double4 [ref_len][levels]     (A_re, A_im, B_re, B_im)
-//   lr   float  [ref_len][levels]      log2 r for the FP32-mantissa engine
-//                                      (-inf = invalid)
-//   c32  Coef32 [ref_len][levels]      A and B as float32 mantissa pairs with
-//                                      an int32 exponent each
-// The radii of one k never grow with d (r_d = min(r_{d-1}, ...), and an
-// invalid entry makes every higher level invalid), so "the highest valid
-// level" is found by binary search; it equals the reference's top-down scan.
+// math_core/high_precision.py:1100-1238; the reference stores
+// float64[levels][ref_len][5]).  Here one allocation holds
+//   coef double4 [levels][ref_len]   (A_re, A_im, B_re, B_im), level-major:
+//                                    the merge of level d reads level d-1 at k
+//                                    and k + 2^(d-1) with coalesced loads
+//   r2m  double  [levels][ref_len]   r^2, level-major (build state)
+//   r2   double  [ref_len][levels]   r^2 as per-k rows (0 = invalid)
+//   r1   double  [ref_len][levels]   sqrt(r^2), the value the reference
+//                                    computes at lookup when |dz| is tiny
+//   lr   float   [ref_len][levels]   log2 r for the FP32-mantissa engine
+//                                    (-inf = invalid)
+//   c32  Coef32  [levels][ref_len]   A and B as float32 mantissa pairs with an
+//                                    int32 exponent each
+// Lookups read one per-k radius row (the level search of one lane touches
+// one 128-byte line, not `levels` scattered words) and gather one
+// coefficient entry.  The radii of one k never grow with d
+// (r_d = min(r_{d-1}, ...), and an invalid entry makes every higher level
+// invalid), so "the highest valid level" is found by binary search; it equals
+// the reference's top-down scan.
 #pragma once
 
 #include "ap_device.hpp"
 
 namespace ap {
 
-struct Coef32 {
+// 32 bytes, so an entry's address is a shift of its index.
+struct __attribute__((aligned(32))) Coef32 {
     float ar, ai, br, bi;
-    int ae, be;
+    int ae, be, pad0, pad1;
 };
 
 struct BlaView {
@@ -37,6 +42,10 @@ struct BlaView {
     const Coef32* c32;
     int levels;   // 0 = no table
     int ref_len;
+    // Tables stay below 2^31 entries (ap_bla_build checks), so 32-bit index
+    // arithmetic suffices.
+    __device__ double4 coef_at(int k, int d) const { return coef[(unsigned)(d * ref_len + k)]; }
+    __device__ Coef32 c32_at(int k, int d) const { return c32[(unsigned)(d * ref_len + k)]; }
 };
 
 __host__ __device__ inline size_t bla_entries(int ref_len, int levels) {
@@ -47,16 +56,17 @@ __host__ __device__ inline size_t align256(size_t b) { return (b + 255) & ~(size
 
 // Byte offsets of the regions inside one table allocation.
 struct BlaLayout {
-    size_t r2, r1, coef, lr, c32, total;
+    size_t coef, r2m, r2, r1, lr, c32, total;
 };
 
 __host__ __device__ inline BlaLayout bla_layout(int ref_len, int levels) {
     size_t n = bla_entries(ref_len, levels);
     BlaLayout L;
-    L.r2 = 0;
+    L.coef = 0;
+    L.r2m = L.coef + align256(n * sizeof(double4));
+    L.r2 = L.r2m + align256(n * sizeof(double));
     L.r1 = L.r2 + align256(n * sizeof(double));
-    L.coef = L.r1 + align256(n * sizeof(double));
-    L.lr = L.coef + align256(n * sizeof(double4));
+    L.lr = L.r1 + align256(n * sizeof(double));
     L.c32 = L.lr + align256(n * sizeof(float));
     L.total = L.c32 + align256(n * sizeof(Coef32));
     return L;

@@ -194,19 +194,23 @@ __device__ Sample pert64_sample(const ap_view& v, const PertData& D, double si, 
         }
         if (has_bla && k < ref_len - 1) {
             const double dz_mag2 = dzx * dzx + dzy * dzy;
-            const size_t row = (size_t)k * D.bla.levels;
+            const unsigned row = (unsigned)(k * D.bla.levels);
             int max_d = D.bla.levels - 1;
             if (dz_mag2 == 0.0 && (dzx != 0.0 || dzy != 0.0)) max_d = max_d < 8 ? max_d : 8;
             const int hi = min(max_d, level_limit_by_length(k, ref_len));
             // Level 0 and the top level are read together; the search runs
             // only when the top level is not valid.
+            // The top level's coefficients are fetched with the radii: in deep
+            // views the top level is the one taken, and the leap then costs a
+            // single round trip to the cache.
             const double r0 = D.bla.r2[row], rtop = D.bla.r2[row + hi];
+            const double4 ctop = D.bla.coef_at(k, hi);
             if (dz_mag2 < r0) {
                 int best = dz_mag2 < rtop ? hi
                                           : highest_valid_level(hi - 1, [&](int d) {
                                                 return dz_mag2 < D.bla.r2[row + d];
                                             });
-                double4 c = D.bla.coef[row + best];
+                double4 c = best == hi ? ctop : D.bla.coef_at(k, best);
                 double ndx = (c.x * dzx - c.y * dzy) + (c.z * dpx - c.w * dpy);
                 dzy = (c.x * dzy + c.y * dzx) + (c.z * dpy + c.w * dpx);
                 dzx = ndx;
@@ -270,6 +274,53 @@ struct EngPert64 {
 };
 
 // =================================================================== FEXP64
+// One BLA leap of the FP64 FloatExp engine from iteration k, if level 0 is
+// valid (math_core/cpu_kernels.py:677-733): returns the number of iterations
+// advanced (0 = no leap).  The radius test is the reference's: |dz|^2 < r^2
+// when the shift is >= -510, else |dz_m| 2^max(-1020, shift) < r with r read
+// from the table.  Level 0 and the top level are tested with one round trip,
+// the top level's coefficients fetched alongside.
+__device__ inline int bla_leap64(const PertData& D, double& dzx, double& dzy, int& z_exp,
+                                 double dpx, double dpy, int k, int E, int ref_len) {
+    const int shift = z_exp - E;
+    const unsigned row = (unsigned)(k * D.bla.levels);
+    const int hi = min(D.bla.levels - 1, level_limit_by_length(k, ref_len));
+    bool can0, cantop;
+    double dz_mag2 = 0.0, tiny = 0.0;
+    if (shift >= -510) {
+        double sc = pow2_f64(shift);
+        dz_mag2 = (dzx * dzx + dzy * dzy) * (sc * sc);
+        const double r0 = D.bla.r2[row], rtop = D.bla.r2[row + hi];
+        can0 = dz_mag2 < r0;
+        cantop = dz_mag2 < rtop;
+    } else {
+        double dz_mag = sqrt(dzx * dzx + dzy * dzy);
+        tiny = dz_mag * pow2_f64(shift > -1020 ? shift : -1020);
+        const double r0 = D.bla.r2[row], rtop = D.bla.r2[row + hi];
+        const double s0 = D.bla.r1[row], stop = D.bla.r1[row + hi];
+        can0 = r0 > 0.0 && tiny < s0;
+        cantop = rtop > 0.0 && tiny < stop;
+    }
+    const double4 ctop = D.bla.coef_at(k, hi);
+    if (!can0) return 0;
+    int best;
+    if (cantop)
+        best = hi;
+    else if (shift >= -510)
+        best = highest_valid_level(hi - 1, [&](int d) { return dz_mag2 < D.bla.r2[row + d]; });
+    else
+        best = highest_valid_level(hi - 1, [&](int d) {
+            return D.bla.r2[row + d] > 0.0 && tiny < D.bla.r1[row + d];
+        });
+    double4 c = best == hi ? ctop : D.bla.coef_at(k, best);
+    double scale_c = (z_exp >= -1020 && z_exp <= 1020) ? pow2_f64(-z_exp) : 0.0;
+    double dxs = dpx * scale_c, dys = dpy * scale_c;
+    double ndx = (c.x * dzx - c.y * dzy) + (c.z * dxs - c.w * dys);
+    dzy = (c.x * dzy + c.y * dzx) + (c.z * dys + c.w * dxs);
+    dzx = ndx;
+    return 1 << best;
+}
+
 template <int T>
 __device__ Sample fexp64_sample(const ap_view& v, const PertData& D, double si, double sj) {
     const bool julia = (T == 2 || T == 3);
@@ -288,6 +339,25 @@ __device__ Sample fexp64_sample(const ap_view& v, const PertData& D, double si, 
     const double kr = v.gen_kr, ki = v.gen_ki;
 
     while (k < ref_len) {
+        // Deep regime (eff < -50): the reference's escape test (eff >= -50) and
+        // the Pauldelbrot test (returns 0 below eff = -40) are both skipped, so
+        // consecutive BLA leaps run in a loop of their own.
+        if ((T == 0 || T == 2 || T == 4) && has_bla) {
+            while (z_exp - E < -50 && k < ref_len - 1) {
+                int step = bla_leap64(D, dzx, dzy, z_exp, dpx, dpy, k, E, ref_len);
+                if (step == 0) break;
+                ++nbla;
+                if (isnan(dzx) || isnan(dzy) || isinf(dzx) || isinf(dzy)) {
+                    Sample r = nan_exit(k);
+                    r.plain = plain;
+                    r.bla = nbla;
+                    return r;
+                }
+                renorm64(dzx, dzy, z_exp);
+                k += step;
+            }
+            if (k >= ref_len) break;
+        }
         const double rx = D.zr[k], ry = D.zi[k];
         const int eff = z_exp - E;
         if (eff >= -50) {
@@ -309,43 +379,8 @@ __device__ Sample fexp64_sample(const ap_view& v, const PertData& D, double si, 
         }
         if (T == 0 || T == 2 || T == 4) {
             if (has_bla && k < ref_len - 1) {
-                const int shift = z_exp - E;
-                const size_t row = (size_t)k * D.bla.levels;
-                const int hi = min(D.bla.levels - 1, level_limit_by_length(k, ref_len));
-                bool can0, cantop;
-                double dz_mag2 = 0.0, dz_mag = 0.0, tiny = 0.0;
-                // Level 0 and the top level are read together; the search runs
-                // only when the top level is not valid.
-                if (shift >= -510) {
-                    double sc = pow2_f64(shift);
-                    dz_mag2 = (dzx * dzx + dzy * dzy) * (sc * sc);
-                    const double r0 = D.bla.r2[row], rtop = D.bla.r2[row + hi];
-                    can0 = dz_mag2 < r0;
-                    cantop = dz_mag2 < rtop;
-                } else {
-                    dz_mag = sqrt(dzx * dzx + dzy * dzy);
-                    tiny = dz_mag * pow2_f64(shift > -1020 ? shift : -1020);
-                    const double r0 = D.bla.r2[row], rtop = D.bla.r2[row + hi];
-                    const double s0 = D.bla.r1[row], stop = D.bla.r1[row + hi];
-                    can0 = r0 > 0.0 && tiny < s0;
-                    cantop = rtop > 0.0 && tiny < stop;
-                }
-                if (can0) {
-                    int best;
-                    if (cantop)
-                        best = hi;
-                    else if (shift >= -510)
-                        best = highest_valid_level(hi - 1, [&](int d) { return dz_mag2 < D.bla.r2[row + d]; });
-                    else
-                        best = highest_valid_level(hi - 1, [&](int d) {
-                            return D.bla.r2[row + d] > 0.0 && tiny < D.bla.r1[row + d];
-                        });
-                    double4 c = D.bla.coef[row + best];
-                    double scale_c = (z_exp >= -1020 && z_exp <= 1020) ? pow2_f64(-z_exp) : 0.0;
-                    double dxs = dpx * scale_c, dys = dpy * scale_c;
-                    double ndx = (c.x * dzx - c.y * dzy) + (c.z * dxs - c.w * dys);
-                    dzy = (c.x * dzy + c.y * dzx) + (c.z * dys + c.w * dxs);
-                    dzx = ndx;
+                int step = bla_leap64(D, dzx, dzy, z_exp, dpx, dpy, k, E, ref_len);
+                if (step > 0) {
                     ++nbla;
                     if (isnan(dzx) || isnan(dzy) || isinf(dzx) || isinf(dzy)) {
                         Sample r = nan_exit(k);
@@ -354,7 +389,7 @@ __device__ Sample fexp64_sample(const ap_view& v, const PertData& D, double si, 
                         return r;
                     }
                     renorm64(dzx, dzy, z_exp);
-                    k += 1 << best;
+                    k += step;
                     continue;
                 }
             }
@@ -473,14 +508,73 @@ struct Fexp32Data {
     BlaView bla;
 };
 
+// floor(log2 m) of a finite, normal, positive float from its exponent field.
+__device__ inline int exponent_of(float m) { return __builtin_amdgcn_frexp_expf(m) - 1; }
+
 __device__ inline void renorm32(float& dzx, float& dzy, int& z_exp) {
     float m = fmaxf(fabsf(dzx), fabsf(dzy));
     if ((m > 4294967296.0f || m < 2.3283064365386963e-10f) && m > 0.0f && isfinite(m)) {
-        int e = ilogbf(m);
+        int e = exponent_of(m);
         dzx = ldexpf(dzx, -e);
         dzy = ldexpf(dzy, -e);
         z_exp += e;
     }
+}
+
+// BLA validity of the FP32 engine: log2|dz| < lr, where |dz|^2 = m2 2^(2 shift)
+// and lr = log2 r.  With e = floor(log2 m2), log2|dz| lies in
+// [e/2 + shift, e/2 + shift + 1/2]; only when lr falls inside that band is
+// the logarithm evaluated.
+struct Ldz {
+    float lo;   // e/2 + shift
+    float m2;
+    int shift;
+};
+
+__device__ inline Ldz make_ldz(float m2, int shift) {
+    Ldz l;
+    l.m2 = m2;
+    l.shift = shift;
+    l.lo = m2 > 0.0f ? 0.5f * (float)(exponent_of(m2) + 2 * shift) : -INFINITY;
+    return l;
+}
+
+__device__ inline bool ldz_below(const Ldz& l, float lr) {
+    if (l.lo + 0.5f < lr) return true;
+    if (l.lo > lr || l.m2 == 0.0f) return l.m2 == 0.0f && lr > -INFINITY;
+    return 0.5f * __log2f(l.m2) + (float)l.shift < lr;
+}
+
+// One BLA leap of the FP32 engine from iteration k, if level 0 is valid:
+// returns the number of iterations advanced (0 = no leap).  dz' = A dz + B dc
+// with A = am 2^ae and B = bm 2^be: the terms have exponents e1 = z_exp + ae
+// (A dz) and e2 = be (B dc, dc in units of 2^-E); the larger becomes the new
+// z_exp and the other term is shifted down.  The top level's coefficients
+// are fetched with the radii: in deep views the top level is the one taken.
+__device__ inline int bla_leap32(const Fexp32Data& D, float& dzx, float& dzy, int& z_exp,
+                                 float dpx, float dpy, int k, int E, int ref_len) {
+    const int L = D.bla.levels;
+    const unsigned row = (unsigned)(k * L);
+    const int hi = min(L - 1, level_limit_by_length(k, ref_len));
+    const float l0 = D.bla.lr[row], ltop = D.bla.lr[row + hi];
+    const Coef32 ctop = D.bla.c32_at(k, hi);
+    const Ldz ldz = make_ldz(dzx * dzx + dzy * dzy, z_exp - E);
+    if (!ldz_below(ldz, l0)) return 0;
+    int best = ldz_below(ldz, ltop) ? hi
+                                    : highest_valid_level(hi - 1, [&](int d) {
+                                          return ldz_below(ldz, D.bla.lr[row + d]);
+                                      });
+    const Coef32 c = best == hi ? ctop : D.bla.c32_at(k, best);
+    float tr = c.ar * dzx - c.ai * dzy, ti = c.ar * dzy + c.ai * dzx;
+    float ur = c.br * dpx - c.bi * dpy, ui = c.br * dpy + c.bi * dpx;
+    int e1 = z_exp + c.ae, e2 = c.be;
+    if (tr == 0.0f && ti == 0.0f) e1 = e2;
+    if (ur == 0.0f && ui == 0.0f) e2 = e1;
+    int ne = max(e1, e2);
+    dzx = ldexpf(tr, e1 - ne) + ldexpf(ur, e2 - ne);
+    dzy = ldexpf(ti, e1 - ne) + ldexpf(ui, e2 - ne);
+    z_exp = ne;
+    return 1 << best;
 }
 
 // Hand-off point of the FP32 loop: once |dz| >= 2^kTail32 the delta continues
@@ -589,6 +683,25 @@ __device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si
     const float kr = (float)v.gen_kr, ki = (float)v.gen_ki;
 
     while (k < ref_len) {
+        // Deep regime: max|dz_m| < 2^32, so |dz| < 2^(eff + 32) < 2^(kTail32 - 8):
+        // no hand-off, no glitch test (it needs eff >= -40) and no escape can
+        // happen; only BLA leaps run, in a loop of their own.
+        if ((T == 0 || T == 2 || T == 4) && has_bla) {
+            while (z_exp - E < kTail32 - 40 && k < ref_len - 1) {
+                int step = bla_leap32(D, dzx, dzy, z_exp, dpx, dpy, k, E, ref_len);
+                if (step == 0) break;
+                ++nbla;
+                if (!isfinite(dzx) || !isfinite(dzy)) {
+                    Sample r = nan_exit(k);
+                    r.plain = plain;
+                    r.bla = nbla;
+                    return r;
+                }
+                renorm32(dzx, dzy, z_exp);
+                k += step;
+            }
+            if (k >= ref_len) break;
+        }
         const int eff = z_exp - E;
         // |dz| = max|dz_m| 2^eff >= 2^kTail32: continue in FP64.
         if (eff >= kTail32 - 40) {
@@ -610,32 +723,8 @@ __device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si
         }
         if (T == 0 || T == 2 || T == 4) {
             if (has_bla && k < ref_len - 1) {
-                const size_t row = (size_t)k * D.bla.levels;
-                // log2 |dz| in actual units.
-                float m2 = dzx * dzx + dzy * dzy;
-                float ldz = m2 > 0.0f ? 0.5f * __log2f(m2) + (float)(z_exp - E) : -INFINITY;
-                const int hi = min(D.bla.levels - 1, level_limit_by_length(k, ref_len));
-                // Level 0 and the top level are read together; the search runs
-                // only when the top level is not valid.
-                const float l0 = D.bla.lr[row], ltop = D.bla.lr[row + hi];
-                if (ldz < l0) {
-                    int best = ldz < ltop ? hi
-                                          : highest_valid_level(hi - 1, [&](int d) {
-                                                return ldz < D.bla.lr[row + d];
-                                            });
-                    const Coef32 c = D.bla.c32[row + best];
-                    // dz' = A dz + B dc: exponents e1 = z_exp + ae (A dz) and
-                    // e2 = be (B dc, dc in units of 2^-E); the larger one becomes
-                    // the new z_exp and the other term is shifted down.
-                    float tr = c.ar * dzx - c.ai * dzy, ti = c.ar * dzy + c.ai * dzx;
-                    float ur = c.br * dpx - c.bi * dpy, ui = c.br * dpy + c.bi * dpx;
-                    int e1 = z_exp + c.ae, e2 = c.be;
-                    if (tr == 0.0f && ti == 0.0f) e1 = e2;
-                    if (ur == 0.0f && ui == 0.0f) e2 = e1;
-                    int ne = max(e1, e2);
-                    dzx = ldexpf(tr, e1 - ne) + ldexpf(ur, e2 - ne);
-                    dzy = ldexpf(ti, e1 - ne) + ldexpf(ui, e2 - ne);
-                    z_exp = ne;
+                int step = bla_leap32(D, dzx, dzy, z_exp, dpx, dpy, k, E, ref_len);
+                if (step > 0) {
                     ++nbla;
                     if (!isfinite(dzx) || !isfinite(dzy)) {
                         Sample r = nan_exit(k);
@@ -644,7 +733,7 @@ __device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si
                         return r;
                     }
                     renorm32(dzx, dzy, z_exp);
-                    k += 1 << best;
+                    k += step;
                     continue;
                 }
             }
