@@ -61,6 +61,9 @@ int grow(ap_ctx* c, void** buf, size_t* have, size_t need, const char* what) {
     return r;
 }
 
+// Device lookups index the table with 32-bit byte offsets (32-byte entries).
+constexpr long long kMaxBlaEntries = 1LL << 27;
+
 int check_view(ap_ctx* c, const ap_view* v, const ap_outputs* o) {
     if (!v || !o) return fail(c, AP_ERR_ARG, "null view or outputs");
     if (v->width <= 0 || v->height <= 0) return fail(c, AP_ERR_ARG, "bad size %dx%d", v->width, v->height);
@@ -93,6 +96,8 @@ int check_orbit(ap_ctx* c, const ap_view* v, const ap_orbit* orb, const void* bl
     if (v->bla_levels > 1) {
         if (!bla) return fail(c, AP_ERR_ARG, "bla_levels=%d but no table", v->bla_levels);
         if (v->bla_levels > 31) return fail(c, AP_ERR_ARG, "bla_levels=%d > 31", v->bla_levels);
+        if ((long long)orb->len * v->bla_levels > kMaxBlaEntries)
+            return fail(c, AP_ERR_ARG, "BLA table of %d x %d entries exceeds 2^27", orb->len, v->bla_levels);
     }
     return AP_OK;
 }
@@ -233,6 +238,12 @@ static int render_fexp(ap_ctx* c, int mant, const ap_view* v, const ap_orbit* or
     if ((r = check_orbit(c, v, orb, bla))) return r;
     if ((r = use_device(c))) return r;
     if (mant == 32) {
+        // The FP32 engine keys |dz|^2 with its exponent in 16 bits (dz_key).
+        if (v->e_scale > 30000 || v->e_scale < -30000)
+            return fail(c, AP_ERR_UNSUPPORTED, "fexp32: e_scale %d outside [-30000, 30000]", v->e_scale);
+        // Its table rows are indexed with 24-bit multiplies.
+        if (v->bla_levels > 1 && orb->len >= (1 << 24))
+            return fail(c, AP_ERR_UNSUPPORTED, "fexp32: BLA orbit of %d iterations exceeds 2^24", orb->len);
         size_t need = (size_t)(orb->len + 1) * 8;
         if ((r = grow(c, &c->orbit32, &c->orbit32_bytes, need, "orbit32 scratch"))) return r;
     }
@@ -260,6 +271,8 @@ int ap_bla_build(ap_ctx* c, const ap_orbit* orb, int32_t type, int32_t n, double
     if (!c) return AP_ERR_ARG;
     if (!orb || !orb->re || !orb->im || !table) return fail(c, AP_ERR_ARG, "null orbit or table");
     if (levels < 2 || levels > 31) return fail(c, AP_ERR_ARG, "levels=%d outside [2, 31]", levels);
+    if ((long long)orb->len * levels > kMaxBlaEntries)
+        return fail(c, AP_ERR_ARG, "BLA table of %d x %d entries exceeds 2^27", orb->len, levels);
     if (type != AP_MANDELBROT && type != AP_JULIA_MANDELBROT && type != AP_GENERAL_MANDELBROT)
         return fail(c, AP_ERR_UNSUPPORTED, "no BLA for fractal type %d", type);
     if (orb->len < 4 || !(tol > 0.0)) return fail(c, AP_ERR_ARG, "ref_len=%d or tol=%g too small", orb->len, tol);

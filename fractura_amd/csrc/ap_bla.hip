@@ -16,8 +16,7 @@ struct BlaMut {
     double* r2m;    // [levels][ref_len]
     double* r2;     // [ref_len][levels]
     double* r1;     // [ref_len][levels]
-    float* lr;      // [ref_len][levels]
-    Coef32* c32;    // [levels][ref_len]
+    Coef32* c32;    // [ref_len][levels]
     int levels;
     int ref_len;
 };
@@ -30,7 +29,6 @@ static BlaMut bla_mut(void* table, int ref_len, int levels) {
     b.r2m = (double*)(p + L.r2m);
     b.r2 = (double*)(p + L.r2);
     b.r1 = (double*)(p + L.r1);
-    b.lr = (float*)(p + L.lr);
     b.c32 = (Coef32*)(p + L.c32);
     b.levels = levels;
     b.ref_len = ref_len;
@@ -135,13 +133,29 @@ __global__ __launch_bounds__(256) void bla_merge_kernel(BlaMut t, int d) {
 }
 
 // Derived regions.  A block takes 16 consecutive k for every level: reads the
-// level-major radii coalesced, transposes them through LDS and writes the
-// per-k rows (r^2, sqrt(r^2) for the FP64 engines' tiny-delta test, log2 r for
-// the FP32 engine); the FP32 coefficient forms stay level-major.
+// level-major build state coalesced, transposes it through LDS and writes the
+// per-k rows (r^2 and sqrt(r^2) for the FP64 engines, the keyed FP32 entry
+// for the FP32-mantissa engine).
 constexpr int kDeriveK = 16;
 
+__device__ inline Coef32 make_coef32(double4 c, double r2) {
+    Coef32 q;
+    double am = fmax(fabs(c.x), fabs(c.y));
+    double bm = fmax(fabs(c.z), fabs(c.w));
+    q.ae = am > 0.0 ? ilogb(am) : kZeroExp;
+    q.be = bm > 0.0 ? ilogb(bm) : kZeroExp;
+    q.ar = am > 0.0 ? (float)ldexp(c.x, -q.ae) : 0.0f;
+    q.ai = am > 0.0 ? (float)ldexp(c.y, -q.ae) : 0.0f;
+    q.br = bm > 0.0 ? (float)ldexp(c.z, -q.be) : 0.0f;
+    q.bi = bm > 0.0 ? (float)ldexp(c.w, -q.be) : 0.0f;
+    q.key = r2_key(r2);
+    q.pad = 0;
+    return q;
+}
+
 __global__ __launch_bounds__(256) void bla_derive_kernel(BlaMut t) {
-    __shared__ double tile[32][kDeriveK + 1];
+    __shared__ double rt[32][kDeriveK + 1];
+    __shared__ Coef32 ct[32][kDeriveK + 1];
     const int L = t.levels;
     const size_t N = (size_t)t.ref_len;
     const int k0 = blockIdx.x * kDeriveK;
@@ -150,18 +164,9 @@ __global__ __launch_bounds__(256) void bla_derive_kernel(BlaMut t) {
         int k = k0 + kl;
         if (k < t.ref_len) {
             size_t e = (size_t)d * N + k;
-            tile[d][kl] = t.r2m[e];
-            double4 c = t.coef[e];
-            Coef32 q;
-            double am = fmax(fabs(c.x), fabs(c.y));
-            double bm = fmax(fabs(c.z), fabs(c.w));
-            q.ae = am > 0.0 ? ilogb(am) : 0;
-            q.be = bm > 0.0 ? ilogb(bm) : 0;
-            q.ar = (float)ldexp(c.x, -q.ae);
-            q.ai = (float)ldexp(c.y, -q.ae);
-            q.br = (float)ldexp(c.z, -q.be);
-            q.bi = (float)ldexp(c.w, -q.be);
-            t.c32[e] = q;
+            double r2 = t.r2m[e];
+            rt[d][kl] = r2;
+            ct[d][kl] = make_coef32(t.coef[e], r2);
         }
     }
     __syncthreads();
@@ -169,11 +174,11 @@ __global__ __launch_bounds__(256) void bla_derive_kernel(BlaMut t) {
         int kl = i / L, d = i % L;
         int k = k0 + kl;
         if (k < t.ref_len) {
-            double r2 = tile[d][kl];
+            double r2 = rt[d][kl];
             size_t row = (size_t)k * L + d;
             t.r2[row] = r2;
             t.r1[row] = sqrt(r2);
-            t.lr[row] = r2 > 0.0 ? (float)(0.5 * log2(r2)) : -INFINITY;
+            t.c32[row] = ct[d][kl];
         }
     }
 }

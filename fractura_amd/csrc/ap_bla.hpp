@@ -12,40 +12,62 @@
 //   r2   double  [ref_len][levels]   r^2 as per-k rows (0 = invalid)
 //   r1   double  [ref_len][levels]   sqrt(r^2), the value the reference
 //                                    computes at lookup when |dz| is tiny
-//   lr   float   [ref_len][levels]   log2 r for the FP32-mantissa engine
-//                                    (-inf = invalid)
-//   c32  Coef32  [levels][ref_len]   A and B as float32 mantissa pairs with an
-//                                    int32 exponent each
-// Lookups read one per-k radius row (the level search of one lane touches
-// one 128-byte line, not `levels` scattered words) and gather one
-// coefficient entry.  The radii of one k never grow with d
+//   c32  Coef32  [ref_len][levels]   the FP32-mantissa engine's entry: A and B
+//                                    as float32 mantissa pairs with an int32
+//                                    exponent each, and r^2 as an ordered
+//                                    int32 key (one 32-byte load per leap)
+// Lookups read one per-k row (the level search of one lane touches one row,
+// not `levels` scattered words).  The radii of one k never grow with d
 // (r_d = min(r_{d-1}, ...), and an invalid entry makes every higher level
 // invalid), so "the highest valid level" is found by binary search; it equals
-// the reference's top-down scan.
+// the reference's top-down scan.  Entries past the end of the orbit
+// (k + 2^d >= ref_len) are invalid in the table itself.
 #pragma once
 
 #include "ap_device.hpp"
 
 namespace ap {
 
-// 32 bytes, so an entry's address is a shift of its index.
+// 32 bytes, so an entry's address is a shift of its index.  A = (ar, ai)
+// 2^ae and B = (br, bi) 2^be with max(|re|, |im|) of each pair in [1, 2);
+// a zero pair has exponent kZeroExp, so it never decides an alignment.
 struct __attribute__((aligned(32))) Coef32 {
     float ar, ai, br, bi;
-    int ae, be, pad0, pad1;
+    int ae, be;
+    int key;  // r2_key(r^2); kInvalidKey = invalid
+    int pad;
 };
+
+constexpr int kZeroExp = -(1 << 20);
+constexpr int kInvalidKey = (int)0x80000000;
+
+// Ordered int32 key of a positive value x = (1 + f) 2^e: (e + 127) 2^15 plus
+// the top 15 bits of f, i.e. the bits of a float with 15 fraction bits and an
+// exponent field extended past 8 bits.  Keys compare as the truncated values
+// do, so key(x) < key(r^2) implies x < r^2 (the test errs on the side of no
+// leap, by at most 2^-15 relative).  Valid for |e| < 2^15 - 128.
+__host__ __device__ inline int r2_key(double r2) {
+    if (!(r2 > 0.0) || !(r2 < 1.0e300)) return r2 > 0.0 ? 0x7fffffff : kInvalidKey;
+    int e = ilogb(r2);
+    double f = ldexp(r2, -e) - 1.0;  // [0, 1), exact
+    return (int)((unsigned)(e + 127) << 15) + (int)(f * 32768.0);
+}
+
+// Key of |dz|^2 = q 2^(2 eff) for a positive normal float q (or q = 0, which
+// keys below every valid radius): the float's bits without the 8 low
+// fraction bits, the exponent field raised by 2 eff.  |eff| < 2^15 - 256.
+__device__ inline int dz_key(float q, int eff) { return (__float_as_int(q) >> 8) + (eff << 16); }
 
 struct BlaView {
     const double* r2;
     const double* r1;
     const double4* coef;
-    const float* lr;
     const Coef32* c32;
     int levels;   // 0 = no table
     int ref_len;
     // Tables stay below 2^31 entries (ap_bla_build checks), so 32-bit index
     // arithmetic suffices.
     __device__ double4 coef_at(int k, int d) const { return coef[(unsigned)(d * ref_len + k)]; }
-    __device__ Coef32 c32_at(int k, int d) const { return c32[(unsigned)(d * ref_len + k)]; }
 };
 
 __host__ __device__ inline size_t bla_entries(int ref_len, int levels) {
@@ -56,7 +78,7 @@ __host__ __device__ inline size_t align256(size_t b) { return (b + 255) & ~(size
 
 // Byte offsets of the regions inside one table allocation.
 struct BlaLayout {
-    size_t coef, r2m, r2, r1, lr, c32, total;
+    size_t coef, r2m, r2, r1, c32, total;
 };
 
 __host__ __device__ inline BlaLayout bla_layout(int ref_len, int levels) {
@@ -66,8 +88,7 @@ __host__ __device__ inline BlaLayout bla_layout(int ref_len, int levels) {
     L.r2m = L.coef + align256(n * sizeof(double4));
     L.r2 = L.r2m + align256(n * sizeof(double));
     L.r1 = L.r2 + align256(n * sizeof(double));
-    L.lr = L.r1 + align256(n * sizeof(double));
-    L.c32 = L.lr + align256(n * sizeof(float));
+    L.c32 = L.r1 + align256(n * sizeof(double));
     L.total = L.c32 + align256(n * sizeof(Coef32));
     return L;
 }
@@ -77,7 +98,6 @@ inline BlaView bla_view(const void* table, int ref_len, int levels) {
     if (!table || levels < 2 || ref_len < 4) {
         b.r2 = b.r1 = nullptr;
         b.coef = nullptr;
-        b.lr = nullptr;
         b.c32 = nullptr;
         b.levels = 0;
         b.ref_len = ref_len;
@@ -88,7 +108,6 @@ inline BlaView bla_view(const void* table, int ref_len, int levels) {
     b.r2 = (const double*)(p + L.r2);
     b.r1 = (const double*)(p + L.r1);
     b.coef = (const double4*)(p + L.coef);
-    b.lr = (const float*)(p + L.lr);
     b.c32 = (const Coef32*)(p + L.c32);
     b.levels = levels;
     b.ref_len = ref_len;

@@ -497,10 +497,10 @@ struct EngFexp64 {
 // =================================================================== FEXP32
 // State: float mantissa pair (dzx, dzy) and int exponent z_exp, actual delta
 // = dz_m * 2^(z_exp - E).  Z_k is read as float2 from an FP32 copy of the
-// orbit.  The mantissa is renormalised to max|dz_m| in [1, 2) by its binary
-// exponent whenever it leaves [2^-32, 2^32] (FP32 has the range to wait).
-// BLA: A and B are float32 mantissas with int exponents; validity compares
-// log2|dz| with log2 r in float32.
+// orbit.  A BLA leap leaves max|dz_m| in [1/2, 1); a plain step renormalises
+// by the binary exponent only when the mantissa leaves [2^-32, 2^32] (FP32
+// has the range to wait).  BLA validity compares the ordered int32 key of
+// |dz|^2 with the key of r^2 stored in the entry.
 struct Fexp32Data {
     const double* zr;
     const double* zi;
@@ -521,60 +521,56 @@ __device__ inline void renorm32(float& dzx, float& dzy, int& z_exp) {
     }
 }
 
-// BLA validity of the FP32 engine: log2|dz| < lr, where |dz|^2 = m2 2^(2 shift)
-// and lr = log2 r.  With e = floor(log2 m2), log2|dz| lies in
-// [e/2 + shift, e/2 + shift + 1/2]; only when lr falls inside that band is
-// the logarithm evaluated.
-struct Ldz {
-    float lo;   // e/2 + shift
-    float m2;
-    int shift;
-};
-
-__device__ inline Ldz make_ldz(float m2, int shift) {
-    Ldz l;
-    l.m2 = m2;
-    l.shift = shift;
-    l.lo = m2 > 0.0f ? 0.5f * (float)(exponent_of(m2) + 2 * shift) : -INFINITY;
-    return l;
+// Row entry idx of the FP32 table.  The 32-bit byte offset (the table holds
+// at most 2^27 entries) lets the load take the table base from scalar
+// registers.
+__device__ inline const Coef32& c32_entry(const Coef32* base, unsigned idx) {
+    return *(const Coef32*)((const char*)base + (idx << 5));
 }
 
-__device__ inline bool ldz_below(const Ldz& l, float lr) {
-    if (l.lo + 0.5f < lr) return true;
-    if (l.lo > lr || l.m2 == 0.0f) return l.m2 == 0.0f && lr > -INFINITY;
-    return 0.5f * __log2f(l.m2) + (float)l.shift < lr;
-}
-
-// One BLA leap of the FP32 engine from iteration k, if level 0 is valid:
-// returns the number of iterations advanced (0 = no leap).  dz' = A dz + B dc
-// with A = am 2^ae and B = bm 2^be: the terms have exponents e1 = z_exp + ae
-// (A dz) and e2 = be (B dc, dc in units of 2^-E); the larger becomes the new
-// z_exp and the other term is shifted down.  The top level's coefficients
-// are fetched with the radii: in deep views the top level is the one taken.
+// One BLA leap of the FP32 engine from iteration k: returns the number of
+// iterations advanced (0 = level 0 not valid).  The entry of the highest
+// level the orbit length allows is fetched together with the key test; in
+// deep views it is the one taken, and the leap costs one round trip to the
+// cache.  dz' = A dz + B dc with A = am 2^ae and B = bm 2^be: the terms have
+// exponents e1 = z_exp + ae and e2 = be (dc in units of 2^-E).  Both inputs
+// are scaled to the larger exponent before the products, which then run as
+// FMAs, and the result is normalised to max|dz_m| in [1/2, 1).  kNoDc: the
+// B dc term is zero (Julia sets), so its exponent takes no part.
+template <bool kNoDc>
 __device__ inline int bla_leap32(const Fexp32Data& D, float& dzx, float& dzy, int& z_exp,
                                  float dpx, float dpy, int k, int E, int ref_len) {
     const int L = D.bla.levels;
-    const unsigned row = (unsigned)(k * L);
+    const unsigned row = __umul24(k, L);  // k < 2^24 (table bound 2^27 entries)
     const int hi = min(L - 1, level_limit_by_length(k, ref_len));
-    const float l0 = D.bla.lr[row], ltop = D.bla.lr[row + hi];
-    const Coef32 ctop = D.bla.c32_at(k, hi);
-    const Ldz ldz = make_ldz(dzx * dzx + dzy * dzy, z_exp - E);
-    if (!ldz_below(ldz, l0)) return 0;
-    int best = ldz_below(ldz, ltop) ? hi
-                                    : highest_valid_level(hi - 1, [&](int d) {
-                                          return ldz_below(ldz, D.bla.lr[row + d]);
-                                      });
-    const Coef32 c = best == hi ? ctop : D.bla.c32_at(k, best);
-    float tr = c.ar * dzx - c.ai * dzy, ti = c.ar * dzy + c.ai * dzx;
-    float ur = c.br * dpx - c.bi * dpy, ui = c.br * dpy + c.bi * dpx;
-    int e1 = z_exp + c.ae, e2 = c.be;
-    if (tr == 0.0f && ti == 0.0f) e1 = e2;
-    if (ur == 0.0f && ui == 0.0f) e2 = e1;
-    int ne = max(e1, e2);
-    dzx = ldexpf(tr, e1 - ne) + ldexpf(ur, e2 - ne);
-    dzy = ldexpf(ti, e1 - ne) + ldexpf(ui, e2 - ne);
-    z_exp = ne;
-    return 1 << best;
+    const Coef32 top = c32_entry(D.bla.c32, row + hi);
+    const int key = dz_key(fmaf(dzx, dzx, dzy * dzy), z_exp - E);
+    int d = hi;
+    if (!(key < top.key)) {
+        if (!(key < c32_entry(D.bla.c32, row).key)) return 0;
+        d = highest_valid_level(hi - 1, [&](int m) { return key < c32_entry(D.bla.c32, row + m).key; });
+    }
+    const Coef32 c = d == hi ? top : c32_entry(D.bla.c32, row + d);
+    const int e1 = z_exp + c.ae;
+    float rx, ry;
+    int ne;
+    if (kNoDc) {
+        ne = e1;
+        rx = fmaf(c.ar, dzx, -c.ai * dzy);
+        ry = fmaf(c.ar, dzy, c.ai * dzx);
+    } else {
+        const int e2 = c.be;
+        ne = max(e1, e2);
+        const float mx = ldexpf(dzx, e1 - ne), my = ldexpf(dzy, e1 - ne);
+        const float px = ldexpf(dpx, e2 - ne), py = ldexpf(dpy, e2 - ne);
+        rx = fmaf(c.ar, mx, fmaf(-c.ai, my, fmaf(c.br, px, -c.bi * py)));
+        ry = fmaf(c.ar, my, fmaf(c.ai, mx, fmaf(c.br, py, c.bi * px)));
+    }
+    const int e = __builtin_amdgcn_frexp_expf(fmaxf(fabsf(rx), fabsf(ry)));
+    dzx = ldexpf(rx, -e);
+    dzy = ldexpf(ry, -e);
+    z_exp = ne + e;
+    return 1 << d;
 }
 
 // Hand-off point of the FP32 loop: once |dz| >= 2^kTail32 the delta continues
@@ -685,19 +681,14 @@ __device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si
     while (k < ref_len) {
         // Deep regime: max|dz_m| < 2^32, so |dz| < 2^(eff + 32) < 2^(kTail32 - 8):
         // no hand-off, no glitch test (it needs eff >= -40) and no escape can
-        // happen; only BLA leaps run, in a loop of their own.
+        // happen; only BLA leaps run, in a loop of their own.  A leap of
+        // finite mantissas (|dz_m| < 2^32, |am|, |bm| < 2, shifts <= 0) is
+        // finite, so the loop has no NaN test.
         if ((T == 0 || T == 2 || T == 4) && has_bla) {
             while (z_exp - E < kTail32 - 40 && k < ref_len - 1) {
-                int step = bla_leap32(D, dzx, dzy, z_exp, dpx, dpy, k, E, ref_len);
+                int step = bla_leap32<julia>(D, dzx, dzy, z_exp, dpx, dpy, k, E, ref_len);
                 if (step == 0) break;
                 ++nbla;
-                if (!isfinite(dzx) || !isfinite(dzy)) {
-                    Sample r = nan_exit(k);
-                    r.plain = plain;
-                    r.bla = nbla;
-                    return r;
-                }
-                renorm32(dzx, dzy, z_exp);
                 k += step;
             }
             if (k >= ref_len) break;
@@ -723,16 +714,9 @@ __device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si
         }
         if (T == 0 || T == 2 || T == 4) {
             if (has_bla && k < ref_len - 1) {
-                int step = bla_leap32(D, dzx, dzy, z_exp, dpx, dpy, k, E, ref_len);
+                int step = bla_leap32<julia>(D, dzx, dzy, z_exp, dpx, dpy, k, E, ref_len);
                 if (step > 0) {
                     ++nbla;
-                    if (!isfinite(dzx) || !isfinite(dzy)) {
-                        Sample r = nan_exit(k);
-                        r.plain = plain;
-                        r.bla = nbla;
-                        return r;
-                    }
-                    renorm32(dzx, dzy, z_exp);
                     k += step;
                     continue;
                 }

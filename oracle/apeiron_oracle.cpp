@@ -555,12 +555,33 @@ Res fexp64_sample(const ap_view& v, const Orbit& o, double si, double sj) {
 struct Coef32 {
     float ar, ai, br, bi;
     int ae, be;
+    int key;
 };
+
+const int kZeroExp = -(1 << 20);
+const int kInvalidKey = (int)0x80000000;
+
+// Ordered int32 key of r^2 = (1 + f) 2^e: (e + 127) 2^15 plus the top 15 bits
+// of f.
+int r2_key(double r2) {
+    if (!(r2 > 0.0) || !(r2 < 1.0e300)) return r2 > 0.0 ? 0x7fffffff : kInvalidKey;
+    int e = ilogb(r2);
+    double f = ldexp(r2, -e) - 1.0;
+    return (int)((unsigned)(e + 127) << 15) + (int)(f * 32768.0);
+}
+
+// Key of |dz|^2 = q 2^(2 eff): the float bits of q without the 8 low fraction
+// bits, the exponent field raised by 2 eff.
+int dz_key(float q, int eff) {
+    int b;
+    memcpy(&b, &q, 4);
+    return (b >> 8) + (int)((unsigned)eff << 16);
+}
 
 struct Orbit32 {
     const Orbit* o;
-    std::vector<float> zfx, zfy, lr;
-    std::vector<Coef32> c32;
+    std::vector<float> zfx, zfy;
+    std::vector<Coef32> c32;  // [ref_len][levels]
 };
 
 void renorm32(float& dzx, float& dzy, int& ze) {
@@ -678,27 +699,37 @@ Res fexp32_sample(const ap_view& v, const Orbit32& O, double si, double sj) {
             }
         }
         if (has_bla && k < ref_len - 1) {
-            float m2 = dzx * dzx + dzy * dzy;
-            float ldz = m2 > 0.0f ? 0.5f * log2f(m2) + (float)(ze - E) : -INFINITY;
-            if (ldz < O.lr[(size_t)k * L]) {
-                int best = 0;
-                for (int d = L - 1; d > 0; --d)
-                    if (k + (1 << d) < ref_len && ldz < O.lr[(size_t)k * L + d]) {
-                        best = d;
-                        break;
-                    }
+            // Highest valid level by a top-down scan of the row (the device
+            // binary-searches the same monotone keys).
+            const int key = dz_key(fmaf(dzx, dzx, dzy * dzy), ze - E);
+            int best = -1;
+            for (int d = L - 1; d >= 0; --d)
+                if (k + (1 << d) < ref_len && key < O.c32[(size_t)k * L + d].key) {
+                    best = d;
+                    break;
+                }
+            if (best >= 0) {
                 const Coef32 c = O.c32[(size_t)k * L + best];
-                float tr = c.ar * dzx - c.ai * dzy, ti = c.ar * dzy + c.ai * dzx;
-                float ur = c.br * dpx - c.bi * dpy, ui = c.br * dpy + c.bi * dpx;
-                int e1 = ze + c.ae, e2 = c.be;
-                if (tr == 0.0f && ti == 0.0f) e1 = e2;
-                if (ur == 0.0f && ui == 0.0f) e2 = e1;
-                int ne = std::max(e1, e2);
-                dzx = ldexpf(tr, e1 - ne) + ldexpf(ur, e2 - ne);
-                dzy = ldexpf(ti, e1 - ne) + ldexpf(ui, e2 - ne);
-                ze = ne;
-                if (!std::isfinite(dzx) || !std::isfinite(dzy)) return nan_res(k);
-                renorm32(dzx, dzy, ze);
+                const int e1 = ze + c.ae;
+                float rx, ry;
+                int ne;
+                if (julia) {
+                    ne = e1;
+                    rx = fmaf(c.ar, dzx, -c.ai * dzy);
+                    ry = fmaf(c.ar, dzy, c.ai * dzx);
+                } else {
+                    const int e2 = c.be;
+                    ne = std::max(e1, e2);
+                    const float mx = ldexpf(dzx, e1 - ne), my = ldexpf(dzy, e1 - ne);
+                    const float px = ldexpf(dpx, e2 - ne), py = ldexpf(dpy, e2 - ne);
+                    rx = fmaf(c.ar, mx, fmaf(-c.ai, my, fmaf(c.br, px, -c.bi * py)));
+                    ry = fmaf(c.ar, my, fmaf(c.ai, mx, fmaf(c.br, py, c.bi * px)));
+                }
+                int e;
+                frexpf(std::max(fabsf(rx), fabsf(ry)), &e);
+                dzx = ldexpf(rx, -e);
+                dzy = ldexpf(ry, -e);
+                ze = ne + e;
                 k += 1 << best;
                 continue;
             }
@@ -1009,22 +1040,20 @@ int or_render_fexp32(const ap_view* v, const double* zr, const double* zi, const
     }
     if (bla) {
         size_t n = (size_t)V.ref_len * levels;
-        O.lr.resize(n);
         O.c32.resize(n);
         for (int k = 0; k < V.ref_len; ++k)
             for (int d = 0; d < levels; ++d) {
                 const double* e = bla + ((size_t)d * V.ref_len + k) * 5;
-                size_t i = (size_t)k * levels + d;
-                O.lr[i] = e[4] > 0.0 ? (float)(0.5 * log2(e[4])) : -INFINITY;
                 double am = std::max(fabs(e[0]), fabs(e[1])), bm = std::max(fabs(e[2]), fabs(e[3]));
                 Coef32 q;
-                q.ae = am > 0.0 ? ilogb(am) : 0;
-                q.be = bm > 0.0 ? ilogb(bm) : 0;
-                q.ar = (float)ldexp(e[0], -q.ae);
-                q.ai = (float)ldexp(e[1], -q.ae);
-                q.br = (float)ldexp(e[2], -q.be);
-                q.bi = (float)ldexp(e[3], -q.be);
-                O.c32[i] = q;
+                q.ae = am > 0.0 ? ilogb(am) : kZeroExp;
+                q.be = bm > 0.0 ? ilogb(bm) : kZeroExp;
+                q.ar = am > 0.0 ? (float)ldexp(e[0], -q.ae) : 0.0f;
+                q.ai = am > 0.0 ? (float)ldexp(e[1], -q.ae) : 0.0f;
+                q.br = bm > 0.0 ? (float)ldexp(e[2], -q.be) : 0.0f;
+                q.bi = bm > 0.0 ? (float)ldexp(e[3], -q.be) : 0.0f;
+                q.key = r2_key(e[4]);
+                O.c32[(size_t)k * levels + d] = q;
             }
     }
     return render(V, *o, true, threads,
