@@ -10,9 +10,12 @@ Workloads (inputs seeded, tests/golden/configs.json):
         reference's FloatExp design) is timed on the same orbit and table.
   fp64  Plain FP64 Mandelbrot at (-0.65, 0), width 4, max_iter 1000,
         3840x2160 (config 1's view at 4K).
-A step is one frame of device work with the inputs resident in HBM: the
-BLA table build (deep) and the pixel pass with the palette colouring fused.
-The host orbit is computed once before timing.
+A step is one frame of device work with the inputs (the reference orbit)
+resident in HBM: the BLA table build from the orbit (deep), the pixel pass
+and the palette colouring, captured as one HIP graph and replayed.  The host
+orbit is computed once before timing.  Eager steps with each part bracketed
+by HIP events on the library's stream give the breakdown; the roofline
+figures use the pass kernel's own event time.
 
 Prints one JSON line.  value = Giter/s, an iteration being one effective
 escape-time iteration of one sample (BLA-skipped iterations count,
@@ -63,6 +66,52 @@ def cpu_info():
     return model, os.cpu_count()
 
 
+class Span:
+    """Context manager recording start/stop events into a Timer (or nothing)."""
+
+    def __init__(self, timer):
+        self.timer = timer
+
+    def __enter__(self):
+        self.ev = self.timer.start() if self.timer else None
+        return self
+
+    def __exit__(self, *exc):
+        if self.timer:
+            self.timer.stop(self.ev)
+        return False
+
+
+def time_graph(torch, graph, warmup, steps):
+    """Wall time per replay of a captured frame (barrier + synchronize on both
+    sides of the timed loop) and the mean event time of one replay."""
+    for _ in range(warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    timer = Timer(torch)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        with Span(timer):
+            graph.replay()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, timer.mean_ms()
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed profiler summary
+    (FETCH_SIZE / WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not t:
+        return None
+    return {"read_bytes": t["read_bytes"], "write_bytes": t["write_bytes"],
+            "bytes": t["read_bytes"] + t["write_bytes"], "source": t["source"]}
+
+
 class Timer:
     """HIP events on torch's current stream, which is the stream the library
     launches on."""
@@ -107,7 +156,7 @@ def run_deep(args, torch):
     assert plan32.engine == "fexp32" and plan64.engine == "fexp64"
     dev = engine.get_device(0)
     dev.set_stream(None)
-    orbit = engine._Orbit(dev, plan32, 0)
+    orbit = engine._Orbit(dev, plan32, 0, forms=_abi.AP_BLA_FORM_ALL)
     lut = engine.get_gpu_cmap_lut("inferno", 2048, False, dev.torch_device)
     n = W * H
     outs = {}
@@ -120,17 +169,23 @@ def run_deep(args, torch):
             rgb=torch.empty((H, W, 3), dtype=torch.uint8, device=dev.torch_device),
             stats=torch.zeros(4, dtype=torch.int64, device=dev.torch_device))
 
-    def step(name, plan, timer=None, with_stats=False):
+    def step(name, plan, timers=None, with_stats=False):
+        """BLA build, pixel pass, colouring: each bracketed by its own HIP
+        events (on the stream the library launches on) when timed."""
         v = plan.view.copy()
-        dev.call("ap_bla_build", C.byref(orbit.abi), v.fractal_type, v.gen_n, v.gen_kr, v.gen_ki,
-                 plan.bla_tol, orbit.levels, engine.ptr(orbit.table))
         o = outs[name]
-        ao = engine._outs(o["counts"], o["smooth"], o["iter_map"], o["glitch"], o["rgb"], lut,
+        ao = engine._outs(o["counts"], o["smooth"], o["iter_map"], o["glitch"],
                           stats=o["stats"] if with_stats else None)
-        s = timer.start() if timer else None
-        engine._launch(dev, name, v, orbit, ao)
-        if timer:
-            timer.stop(s)
+        t = timers or {}
+        with Span(t.get("build")):
+            dev.call("ap_bla_build", C.byref(orbit.abi), v.fractal_type, v.gen_n, v.gen_kr,
+                     v.gen_ki, plan.bla_tol, orbit.levels, engine.bla_forms(name),
+                     engine.ptr(orbit.table))
+        with Span(t.get("pass")):
+            engine._launch(dev, name, v, orbit, ao)
+        with Span(t.get("colour")):
+            dev.call("ap_colorize", C.byref(v), engine.ptr(o["iter_map"]), None, 0,
+                     engine.ptr(lut), engine.ptr(o["rgb"]))
 
     results = {}
     for name, plan in (("fexp32", plan32), ("fexp64", plan64)):
@@ -142,16 +197,23 @@ def run_deep(args, torch):
         for _ in range(args.warmup):
             step(name, plan)
         torch.cuda.synchronize()
-        timer = Timer(torch)
+        # Eager steps, each part bracketed by its own events: the breakdown
+        # and the pass kernel's time for the roofline.
+        timers = {k: Timer(torch) for k in ("build", "pass", "colour")}
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            step(name, plan, timer)
+            step(name, plan, timers)
         torch.cuda.synchronize()
-        wall = (time.perf_counter() - t0) / args.steps
-        kern_ms = timer.mean_ms()
+        eager_wall = (time.perf_counter() - t0) / args.steps
+        # The frame as one HIP graph (build, pass, colouring): the timed steps.
+        wall, graph_ms = time_graph(torch, engine.capture_graph(dev, lambda: step(name, plan)),
+                                    args.warmup, args.steps)
+        kern_ms = timers["pass"].mean_ms()
         flops = plain * FLOPS_FEXP_PLAIN + bla * FLOPS_BLA_STEP
         results[name] = dict(iters=iters, plain=plain, bla=bla, wall_ms=wall * 1e3,
-                             kernel_ms=kern_ms, giters=iters / wall / 1e9,
+                             graph_ms=graph_ms, eager_wall_ms=eager_wall * 1e3,
+                             kernel_ms=kern_ms, build_ms=timers["build"].mean_ms(),
+                             colour_ms=timers["colour"].mean_ms(), giters=iters / wall / 1e9,
                              tflops=flops / (kern_ms * 1e-3) / 1e12 if kern_ms > 0 else 0.0)
         log(f"{name}: {results[name]}")
 
@@ -192,13 +254,30 @@ def run_deep(args, torch):
         "executed_steps": {"plain": r32["plain"], "bla": r32["bla"]},
         "roofline": {"bound": "valu", "kernel": "tile_pass<EngFexp32<0>>",
                      "achieved": r32["tflops"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": r32["tflops"] / FP32_PEAK_TFLOPS, "traffic": None,
-                     "kernel_ms": r32["kernel_ms"]},
+                     "frac": r32["tflops"] / FP32_PEAK_TFLOPS,
+                     "traffic": measured_traffic("tile_pass<EngFexp32<0>>"),
+                     "kernel_ms": r32["kernel_ms"],
+                     "flops_per_step": {"plain": FLOPS_FEXP_PLAIN, "bla": FLOPS_BLA_STEP}},
         "roofline_fp64": {"kernel": "tile_pass<EngFexp64<0>>", "achieved": r64["tflops"],
                           "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                           "frac": r64["tflops"] / FP64_PEAK_TFLOPS, "kernel_ms": r64["kernel_ms"]},
+        "step_ms": {e: {"bla_build": r["build_ms"], "pass": r["kernel_ms"],
+                        "colour": r["colour_ms"], "graph_replay": r["graph_ms"],
+                        "wall": r["wall_ms"], "eager_wall": r["eager_wall_ms"]}
+                    for e, r in (("fexp32", r32), ("fexp64", r64))},
+        "colour_pass": colour_roofline(n, r32["colour_ms"]),
         "cpu_baseline": cpu,
     }
+
+
+def colour_roofline(pixels, ms):
+    """The colouring pass is HBM-bound: it reads the float32 map and writes
+    RGB8, 7 bytes per pixel."""
+    nbytes = pixels * 7
+    gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "hbm", "kernel": "colorize_kernel", "bytes": nbytes, "ms": ms,
+            "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "traffic": measured_traffic("colorize_kernel")}
 
 
 def oracle_block(plan, orbit_host, bla_tol, c64, c32, size=48):
@@ -222,33 +301,41 @@ def oracle_block(plan, orbit_host, bla_tol, c64, c32, size=48):
 
 
 def cpu_baseline_fexp(plan, seconds):
-    """Oracle FloatExp (FP64 mantissa) over a centred tile on all host cores,
-    tile side grown until the run takes about `seconds`."""
+    """Oracle FloatExp (FP64 mantissa) on all host cores: a centred tile of the
+    same view, grown up to the whole frame, then repeated until the sample
+    has run for about `seconds`."""
     from oracle import oracle
 
     v = plan.view.copy()
     re, im, ref_len = plan.orbit
     bla = oracle.bla_build(re, im, ref_len, v.fractal_type, plan.bla_tol, 16)
-    side = 32
     model, cores = cpu_info()
     threads = oracle.load().or_num_threads()
+    side = 32
     while True:
         sub = v.copy()
-        side = min(side, v.width, v.height)
-        x0, y0 = (v.width - side) // 2, (v.height - side) // 2
-        sub.width = sub.height = side
+        tw, th = min(side, v.width), min(side, v.height)
+        x0, y0 = (v.width - tw) // 2, (v.height - th) // 2
+        sub.width, sub.height = tw, th
         sub.ref_px, sub.ref_py = v.ref_px - x0 * v.factor, v.ref_py - y0 * v.factor
         t0 = time.perf_counter()
         m = oracle.render("fexp64", sub, orbit=(re, im), bla=bla, levels=16)
         dt = time.perf_counter() - t0
-        if dt > seconds / 4 or side >= min(v.width, v.height):
+        if dt > seconds / 4 or (tw == v.width and th == v.height):
             break
         side *= 2
-    return {"value": m.iterations / dt / 1e9, "unit": "Giter/s", "cores": threads,
+    iters, total, reps = m.iterations, dt, 1
+    while total < seconds:
+        t0 = time.perf_counter()
+        m = oracle.render("fexp64", sub, orbit=(re, im), bla=bla, levels=16)
+        total += time.perf_counter() - t0
+        iters += m.iterations
+        reps += 1
+    return {"value": iters / total / 1e9, "unit": "Giter/s", "cores": threads,
             "kind": "port", "cpu": model,
-            "sample": f"centred {side}x{side} tile of the same view, FP64-mantissa FloatExp "
-                      f"restatement (oracle/apeiron_oracle.cpp, -O2 -ffp-contract=off, OpenMP "
-                      f"{threads} threads), {dt:.2f}s"}
+            "sample": f"centred {tw}x{th} tile of the same view x {reps} runs, FP64-mantissa "
+                      f"FloatExp restatement with the same BLA table (oracle/apeiron_oracle.cpp, "
+                      f"OpenMP {threads} threads), {total:.1f}s"}
 
 
 def run_fp64(args, torch):
@@ -265,12 +352,14 @@ def run_fp64(args, torch):
     rgb = torch.empty((H, W, 3), dtype=torch.uint8, device=dev.torch_device)
     stats = torch.zeros(4, dtype=torch.int64, device=dev.torch_device)
 
-    def step(timer=None, with_stats=False):
-        o = engine._outs(counts, smooth, it, None, rgb, lut, stats=stats if with_stats else None)
-        s = timer.start() if timer else None
-        engine._launch(dev, "fp64", plan.view, None, o)
-        if timer:
-            timer.stop(s)
+    def step(timers=None, with_stats=False):
+        o = engine._outs(counts, smooth, it, None, stats=stats if with_stats else None)
+        t = timers or {}
+        with Span(t.get("pass")):
+            engine._launch(dev, "fp64", plan.view, None, o)
+        with Span(t.get("colour")):
+            dev.call("ap_colorize", C.byref(plan.view), engine.ptr(it), None, 0, engine.ptr(lut),
+                     engine.ptr(rgb))
 
     step(with_stats=True)
     torch.cuda.synchronize()
@@ -278,26 +367,33 @@ def run_fp64(args, torch):
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    timer = Timer(torch)
+    timers = {k: Timer(torch) for k in ("pass", "colour")}
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step(timer)
+        step(timers)
     torch.cuda.synchronize()
-    wall = (time.perf_counter() - t0) / args.steps
-    kms = timer.mean_ms()
+    eager_wall = (time.perf_counter() - t0) / args.steps
+    wall, graph_ms = time_graph(torch, engine.capture_graph(dev, lambda: step()), args.warmup,
+                                args.steps)
+    kms = timers["pass"].mean_ms()
     tflops = iters * FLOPS_FP64_ITER / (kms * 1e-3) / 1e12
     cpu = None
     if not args.no_cpu:
         from oracle import oracle
         model, _ = cpu_info()
         threads = oracle.load().or_num_threads()
-        t0 = time.perf_counter()
-        m = oracle.render("fp64", plan.view)
-        dt = time.perf_counter() - t0
-        same = bool(np.array_equal(m.counts, counts.cpu().numpy()))
-        cpu = {"value": m.iterations / dt / 1e9, "unit": "Giter/s", "cores": threads, "kind": "port",
-               "cpu": model, "sample": f"full {W}x{H} frame, oracle FP64 restatement, {dt:.2f}s; "
-                                       f"counts bit-exact vs GPU: {same}"}
+        total, its, reps, same = 0.0, 0, 0, True
+        while total < args.cpu_seconds or reps == 0:
+            t0 = time.perf_counter()
+            m = oracle.render("fp64", plan.view)
+            total += time.perf_counter() - t0
+            its += m.iterations
+            reps += 1
+            if reps == 1:
+                same = bool(np.array_equal(m.counts, counts.cpu().numpy()))
+        cpu = {"value": its / total / 1e9, "unit": "Giter/s", "cores": threads, "kind": "port",
+               "cpu": model, "sample": f"full {W}x{H} frame x {reps} runs, oracle FP64 "
+                                       f"restatement, {total:.1f}s; counts bit-exact vs GPU: {same}"}
     return {
         "metric": "Giter/s per GPU, FP32+FloatExp perturb vs FP64 at matched count accuracy",
         "value": iters / wall / 1e9, "unit": "Giter/s", "ms_per_step": wall * 1e3, "dtype": "f64",
@@ -305,7 +401,10 @@ def run_fp64(args, torch):
         "iterations_per_step": iters,
         "roofline": {"bound": "valu", "kernel": "tile_pass<EngFP64<0>>", "achieved": tflops,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS,
-                     "traffic": None, "kernel_ms": kms},
+                     "traffic": measured_traffic("tile_pass<EngFP64<0>>"), "kernel_ms": kms},
+        "step_ms": {"pass": kms, "colour": timers["colour"].mean_ms(), "graph_replay": graph_ms,
+                    "wall": wall * 1e3, "eager_wall": eager_wall * 1e3},
+        "colour_pass": colour_roofline(W * H, timers["colour"].mean_ms()),
         "cpu_baseline": cpu,
     }
 

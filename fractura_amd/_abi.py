@@ -25,6 +25,10 @@ AP_STAT_BLA_STEPS = 2
 AP_STAT_SAMPLES = 3
 AP_STAT_COUNT = 4
 
+AP_BLA_FORM_FP64 = 1
+AP_BLA_FORM_FP32 = 2
+AP_BLA_FORM_ALL = 3
+
 
 class ApView(C.Structure):
     _fields_ = [
@@ -94,7 +98,7 @@ SIGNATURES = {
     "ap_render_fexp64": (C.c_int, [_P, _VIEW, _ORB, _P, _OUT]),
     "ap_render_fexp32": (C.c_int, [_P, _VIEW, _ORB, _P, _OUT]),
     "ap_bla_bytes": (C.c_size_t, [_I, _I]),
-    "ap_bla_build": (C.c_int, [_P, _ORB, _I, _I, _D, _D, _D, _I, _P]),
+    "ap_bla_build": (C.c_int, [_P, _ORB, _I, _I, _D, _D, _D, _I, _I, _P]),
     "ap_bla_export": (C.c_int, [_P, _P, _I, _I, _P]),
     "ap_edge_mask": (C.c_int, [_P, _P, _I, _I, C.c_float, _P]),
     "ap_compact": (C.c_int, [_P, _P, _I, _P, _P]),

@@ -61,9 +61,6 @@ int grow(ap_ctx* c, void** buf, size_t* have, size_t need, const char* what) {
     return r;
 }
 
-// Device lookups index the table with 32-bit byte offsets (32-byte entries).
-constexpr long long kMaxBlaEntries = 1LL << 27;
-
 int check_view(ap_ctx* c, const ap_view* v, const ap_outputs* o) {
     if (!v || !o) return fail(c, AP_ERR_ARG, "null view or outputs");
     if (v->width <= 0 || v->height <= 0) return fail(c, AP_ERR_ARG, "bad size %dx%d", v->width, v->height);
@@ -82,6 +79,9 @@ int check_view(ap_ctx* c, const ap_view* v, const ap_outputs* o) {
         return fail(c, AP_ERR_ARG, "rebase pass needs a glitch mask or a queue");
     if (o->queue && !o->queue_count) return fail(c, AP_ERR_ARG, "queue without a device count");
     if (o->rgb && !o->lut) return fail(c, AP_ERR_ARG, "rgb output needs a LUT");
+    if (o->rgb && ((uintptr_t)o->rgb & 15)) return fail(c, AP_ERR_ARG, "rgb buffer must be 16-byte aligned");
+    if (o->iter_map && ((uintptr_t)o->iter_map & 15))
+        return fail(c, AP_ERR_ARG, "iter_map must be 16-byte aligned");
     if (o->rgb && v->mode == AP_MODE_CENTRE && !o->iter_map)
         return fail(c, AP_ERR_ARG, "a centre pass colours its float32 map: rgb needs iter_map");
     if (o->cdf && o->num_bins < 2) return fail(c, AP_ERR_ARG, "cdf needs num_bins >= 2");
@@ -96,8 +96,9 @@ int check_orbit(ap_ctx* c, const ap_view* v, const ap_orbit* orb, const void* bl
     if (v->bla_levels > 1) {
         if (!bla) return fail(c, AP_ERR_ARG, "bla_levels=%d but no table", v->bla_levels);
         if (v->bla_levels > 31) return fail(c, AP_ERR_ARG, "bla_levels=%d > 31", v->bla_levels);
-        if ((long long)orb->len * v->bla_levels > kMaxBlaEntries)
-            return fail(c, AP_ERR_ARG, "BLA table of %d x %d entries exceeds 2^27", orb->len, v->bla_levels);
+        if (orb->len > ap::kMaxBlaRefLen || (long long)orb->len * v->bla_levels > ap::kMaxBlaEntries)
+            return fail(c, AP_ERR_ARG, "BLA table of %d x %d entries exceeds the limits (ref_len < 2^24, 2^27 entries)",
+                        orb->len, v->bla_levels);
     }
     return AP_OK;
 }
@@ -241,11 +242,12 @@ static int render_fexp(ap_ctx* c, int mant, const ap_view* v, const ap_orbit* or
         // The FP32 engine keys |dz|^2 with its exponent in 16 bits (dz_key).
         if (v->e_scale > 30000 || v->e_scale < -30000)
             return fail(c, AP_ERR_UNSUPPORTED, "fexp32: e_scale %d outside [-30000, 30000]", v->e_scale);
-        // Its table rows are indexed with 24-bit multiplies.
-        if (v->bla_levels > 1 && orb->len >= (1 << 24))
-            return fail(c, AP_ERR_UNSUPPORTED, "fexp32: BLA orbit of %d iterations exceeds 2^24", orb->len);
-        size_t need = (size_t)(orb->len + 1) * 8;
-        if ((r = grow(c, &c->orbit32, &c->orbit32_bytes, need, "orbit32 scratch"))) return r;
+        // Without a table the FP32 orbit copy goes to context scratch; a
+        // table carries its own (written by ap_bla_build's FP32 form).
+        if (!(bla && v->bla_levels > 1 && orb->len >= 4)) {  // the test bla_view applies
+            size_t need = (size_t)(orb->len + 1) * 8;
+            if ((r = grow(c, &c->orbit32, &c->orbit32_bytes, need, "orbit32 scratch"))) return r;
+        }
     }
     return check(c, ap::launch_fexp(mant, *v, *orb, bla, *o, c->orbit32, c->stream),
                  mant == 32 ? "fexp32 pass" : "fexp64 pass");
@@ -267,19 +269,22 @@ size_t ap_bla_bytes(int32_t ref_len, int32_t levels) {
 }
 
 int ap_bla_build(ap_ctx* c, const ap_orbit* orb, int32_t type, int32_t n, double kr, double ki,
-                 double tol, int32_t levels, void* table) {
+                 double tol, int32_t levels, int32_t forms, void* table) {
     if (!c) return AP_ERR_ARG;
     if (!orb || !orb->re || !orb->im || !table) return fail(c, AP_ERR_ARG, "null orbit or table");
     if (levels < 2 || levels > 31) return fail(c, AP_ERR_ARG, "levels=%d outside [2, 31]", levels);
-    if ((long long)orb->len * levels > kMaxBlaEntries)
-        return fail(c, AP_ERR_ARG, "BLA table of %d x %d entries exceeds 2^27", orb->len, levels);
+    if (orb->len > ap::kMaxBlaRefLen || (long long)orb->len * levels > ap::kMaxBlaEntries)
+        return fail(c, AP_ERR_ARG, "BLA table of %d x %d entries exceeds the limits (ref_len < 2^24, 2^27 entries)",
+                    orb->len, levels);
     if (type != AP_MANDELBROT && type != AP_JULIA_MANDELBROT && type != AP_GENERAL_MANDELBROT)
         return fail(c, AP_ERR_UNSUPPORTED, "no BLA for fractal type %d", type);
     if (orb->len < 4 || !(tol > 0.0)) return fail(c, AP_ERR_ARG, "ref_len=%d or tol=%g too small", orb->len, tol);
     if (type == AP_GENERAL_MANDELBROT && (n < 2 || n > 64)) return fail(c, AP_ERR_ARG, "bad gen_n %d", n);
+    if (forms < 1 || forms > AP_BLA_FORM_ALL) return fail(c, AP_ERR_ARG, "bad BLA forms mask %d", forms);
     int r = use_device(c);
     if (r) return r;
-    return check(c, ap::launch_bla_build(*orb, type, n, kr, ki, tol, levels, table, c->stream), "bla build");
+    return check(c, ap::launch_bla_build(*orb, type, n, kr, ki, tol, levels, forms, table, c->stream),
+                 "bla build");
 }
 
 int ap_bla_export(ap_ctx* c, const void* table, int32_t ref_len, int32_t levels, double* out) {
@@ -340,6 +345,8 @@ int ap_colorize(ap_ctx* c, const ap_view* v, const float* it, const float* cdf, 
     if (!v || !it || !lut || !rgb || v->width <= 0 || v->height <= 0)
         return fail(c, AP_ERR_ARG, "bad colorize arguments");
     if (cdf && num_bins < 2) return fail(c, AP_ERR_ARG, "cdf needs num_bins >= 2");
+    if (((uintptr_t)it & 15) || ((uintptr_t)rgb & 15))
+        return fail(c, AP_ERR_ARG, "colorize needs 16-byte aligned map and rgb buffers");
     int r = use_device(c);
     if (r) return r;
     return check(c, ap::launch_colorize(*v, it, cdf, num_bins, lut, rgb, c->stream), "colorize");

@@ -3,9 +3,19 @@
 // Level 0 and the merge rule restate _jit_build_bla_table
 // (math_core/high_precision.py:1100-1238) operation by operation in FP64
 // (contraction off), so A, B and r^2 are bit-identical to the CPU
-// restatement.  Level d only reads level d-1, so each level is one launch
-// over all k; the reference builds the same table serially on the host and
-// copies 16 x ref_len x 40 bytes over PCIe (math_core/engine.py:416-423).
+// restatement.  Level d only reads level d-1.  The reference builds the same
+// table serially on the host and copies 16 x ref_len x 40 bytes over PCIe
+// (math_core/engine.py:416-423).
+//
+// Two kernels:
+//   bla_low_kernel    levels 0..kLowTop in LDS: a block owns kLowOut
+//                     consecutive k and computes level 0 on that span plus
+//                     the 2^kLowTop - 1 entries to its right that the merges
+//                     reach, then each level in place; each level is written
+//                     once, in the requested forms (and level kLowTop also
+//                     as A, B, r^2 for the first merge launch).
+//   bla_merge_kernel  one higher level per launch over all k, reading level
+//                     d-1 from the table.
 #include "ap_bla.hpp"
 #include "ap_launch.hpp"
 
@@ -13,38 +23,50 @@ namespace ap {
 
 struct BlaMut {
     double4* coef;  // [levels][ref_len]
-    double* r2m;    // [levels][ref_len]
-    double* r2;     // [ref_len][levels]
-    double* r1;     // [ref_len][levels]
-    Coef32* c32;    // [ref_len][levels]
+    double* r2;
+    double* r1;
+    Coef32* c32;
+    float2* zf;  // [ref_len + 1]
     int levels;
     int ref_len;
+    int forms;     // AP_BLA_FORM_* bits
+    int state_lo;  // levels [state_lo, levels - 1) are read back by the merges
 };
 
-static BlaMut bla_mut(void* table, int ref_len, int levels) {
+static BlaMut bla_mut(void* table, int ref_len, int levels, int forms, int state_lo) {
     BlaLayout L = bla_layout(ref_len, levels);
     char* p = (char*)table;
     BlaMut b;
     b.coef = (double4*)(p + L.coef);
-    b.r2m = (double*)(p + L.r2m);
     b.r2 = (double*)(p + L.r2);
     b.r1 = (double*)(p + L.r1);
     b.c32 = (Coef32*)(p + L.c32);
+    b.zf = (float2*)(p + L.zf);
     b.levels = levels;
     b.ref_len = ref_len;
+    b.forms = forms;
+    b.state_lo = state_lo;
     return b;
 }
 
-__global__ __launch_bounds__(256) void bla_level0_kernel(BlaMut t, const double* __restrict__ zr,
-                                                         const double* __restrict__ zi, int type,
-                                                         int n, double kr, double ki, double tol) {
-    int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= t.ref_len) return;
-    if (k >= t.ref_len - 1) {
-        t.coef[k] = make_double4(0.0, 0.0, 0.0, 0.0);
-        t.r2m[k] = 0.0;
-        return;
-    }
+struct BlaEntry {
+    double4 c;  // A_re, A_im, B_re, B_im
+    double r2;
+};
+
+__device__ inline BlaEntry bla_zero() {
+    BlaEntry e;
+    e.c = make_double4(0.0, 0.0, 0.0, 0.0);
+    e.r2 = 0.0;
+    return e;
+}
+
+// Level 0 at k: A = dF/dz at Z_k, B = 1, and the validity radius of the
+// reference (math_core/high_precision.py:1127-1185).
+__device__ inline BlaEntry bla_level0(const double* __restrict__ zr, const double* __restrict__ zi,
+                                      int k, int ref_len, int type, int n, double kr, double ki,
+                                      double tol) {
+    if (k >= ref_len - 1) return bla_zero();
     double rx = zr[k], ry = zi[k];
     double r_mag2 = rx * rx + ry * ry;
     double ar, ai, r_sq;
@@ -83,62 +105,46 @@ __global__ __launch_bounds__(256) void bla_level0_kernel(BlaMut t, const double*
         if (r_sq > 0.25) r_sq = 0.25;
         if (k == 0 && r_mag2 < 1e-14) r_sq = 0.0;
     }
-    t.coef[k] = make_double4(ar, ai, 1.0, 0.0);
-    t.r2m[k] = r_sq;
+    BlaEntry e;
+    e.c = make_double4(ar, ai, 1.0, 0.0);
+    e.r2 = r_sq;
+    return e;
 }
 
-// Level d from level d-1, both level-major: lanes k and k+1 read adjacent
-// words, so every load and store is coalesced.
-__global__ __launch_bounds__(256) void bla_merge_kernel(BlaMut t, int d) {
-    int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= t.ref_len) return;
-    const size_t N = (size_t)t.ref_len;
-    const int step = 1 << (d - 1);
-    double4 out = make_double4(0.0, 0.0, 0.0, 0.0);
-    double r_out = 0.0;
-    if (k < t.ref_len - 2 * step) {
-        size_t e1 = (size_t)(d - 1) * N + k;
-        size_t e2 = e1 + step;
-        double r1_sq = t.r2m[e1], r2_sq = t.r2m[e2];
-        if (r1_sq > 0.0 && r2_sq > 0.0) {
-            double4 c1 = t.coef[e1], c2 = t.coef[e2];
-            double ar = c2.x * c1.x - c2.y * c1.y;
-            double ai = c2.x * c1.y + c2.y * c1.x;
-            double br = (c2.x * c1.z - c2.y * c1.w) + c2.z;
-            double bi = (c2.x * c1.w + c2.y * c1.z) + c2.w;
-            double a_mag2 = ar * ar + ai * ai;
-            double b_mag2 = br * br + bi * bi;
-            bool bad = isnan(a_mag2) || isinf(a_mag2) || isnan(b_mag2) || isinf(b_mag2) ||
-                       a_mag2 > 1e150 || b_mag2 > 1e150;
-            if (!bad) {
-                out = make_double4(ar, ai, br, bi);
-                double a1_mag2 = c1.x * c1.x + c1.y * c1.y;
-                double cand;
-                if (a1_mag2 > 1e-300) {
-                    double r2s = (0.25 * r2_sq) / a1_mag2;
-                    cand = r1_sq < r2s ? r1_sq : r2s;
-                } else {
-                    cand = r1_sq;
-                }
-                if (isnan(cand) || isinf(cand) || cand <= 0.0)
-                    r_out = 0.0;
-                else
-                    r_out = cand > 0.25 ? 0.25 : cand;
-            }
-        }
+// Entry (k, d) from e1 = (k, d-1) and e2 = (k + 2^(d-1), d-1): A = A2 A1,
+// B = A2 B1 + B2, r = min(r1, r2 / (2 |A1|)), clamped to 1/4, with the
+// overflow cut at |A|^2, |B|^2 > 1e150 (math_core/high_precision.py:1187-1236).
+// The caller passes e1, e2 only when k + 2^d < ref_len.
+__device__ inline BlaEntry bla_merge(const BlaEntry& e1, const BlaEntry& e2) {
+    BlaEntry out = bla_zero();
+    if (!(e1.r2 > 0.0 && e2.r2 > 0.0)) return out;
+    const double4 c1 = e1.c, c2 = e2.c;
+    double ar = c2.x * c1.x - c2.y * c1.y;
+    double ai = c2.x * c1.y + c2.y * c1.x;
+    double br = (c2.x * c1.z - c2.y * c1.w) + c2.z;
+    double bi = (c2.x * c1.w + c2.y * c1.z) + c2.w;
+    double a_mag2 = ar * ar + ai * ai;
+    double b_mag2 = br * br + bi * bi;
+    bool bad = isnan(a_mag2) || isinf(a_mag2) || isnan(b_mag2) || isinf(b_mag2) ||
+               a_mag2 > 1e150 || b_mag2 > 1e150;
+    if (bad) return out;
+    out.c = make_double4(ar, ai, br, bi);
+    double a1_mag2 = c1.x * c1.x + c1.y * c1.y;
+    double cand;
+    if (a1_mag2 > 1e-300) {
+        double r2s = (0.25 * e2.r2) / a1_mag2;
+        cand = e1.r2 < r2s ? e1.r2 : r2s;
+    } else {
+        cand = e1.r2;
     }
-    size_t e = (size_t)d * N + k;
-    t.coef[e] = out;
-    t.r2m[e] = r_out;
+    if (isnan(cand) || isinf(cand) || cand <= 0.0)
+        out.r2 = 0.0;
+    else
+        out.r2 = cand > 0.25 ? 0.25 : cand;
+    return out;
 }
 
-// Derived regions.  A block takes 16 consecutive k for every level: reads the
-// level-major build state coalesced, transposes it through LDS and writes the
-// per-k rows (r^2 and sqrt(r^2) for the FP64 engines, the keyed FP32 entry
-// for the FP32-mantissa engine).
-constexpr int kDeriveK = 16;
-
-__device__ inline Coef32 make_coef32(double4 c, double r2) {
+__device__ inline Coef32 make_coef32(double4 c, double r2, int next_hi) {
     Coef32 q;
     double am = fmax(fabs(c.x), fabs(c.y));
     double bm = fmax(fabs(c.z), fabs(c.w));
@@ -149,79 +155,139 @@ __device__ inline Coef32 make_coef32(double4 c, double r2) {
     q.br = bm > 0.0 ? (float)ldexp(c.z, -q.be) : 0.0f;
     q.bi = bm > 0.0 ? (float)ldexp(c.w, -q.be) : 0.0f;
     q.key = r2_key(r2);
-    q.pad = 0;
+    q.next_hi = next_hi;
     return q;
 }
 
-__global__ __launch_bounds__(256) void bla_derive_kernel(BlaMut t) {
-    __shared__ double rt[32][kDeriveK + 1];
-    __shared__ Coef32 ct[32][kDeriveK + 1];
-    const int L = t.levels;
-    const size_t N = (size_t)t.ref_len;
-    const int k0 = blockIdx.x * kDeriveK;
-    for (int i = threadIdx.x; i < L * kDeriveK; i += 256) {
-        int d = i / kDeriveK, kl = i % kDeriveK;
-        int k = k0 + kl;
-        if (k < t.ref_len) {
-            size_t e = (size_t)d * N + k;
-            double r2 = t.r2m[e];
-            rt[d][kl] = r2;
-            ct[d][kl] = make_coef32(t.coef[e], r2);
-        }
+// Entry (k, d) in the requested forms, one coalesced store per region.  A and
+// B with r^2 are also kept for the levels a later merge launch reads.
+__device__ inline void store_entry(const BlaMut& t, int k, int d, const BlaEntry& e) {
+    size_t i = (size_t)d * t.ref_len + k;
+    const bool fp64 = (t.forms & AP_BLA_FORM_FP64) != 0;
+    if (fp64 || (d >= t.state_lo && d < t.levels - 1)) {
+        t.coef[i] = e.c;
+        t.r2[i] = e.r2;
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < L * kDeriveK; i += 256) {
-        int kl = i / L, d = i % L;
-        int k = k0 + kl;
-        if (k < t.ref_len) {
-            double r2 = rt[d][kl];
-            size_t row = (size_t)k * L + d;
-            t.r2[row] = r2;
-            t.r1[row] = sqrt(r2);
-            t.c32[row] = ct[d][kl];
+    if (fp64) t.r1[i] = sqrt(e.r2);
+    if (t.forms & AP_BLA_FORM_FP32)
+        t.c32[i] = make_coef32(e.c, e.r2, top_level(k + (1 << d), t.ref_len, t.levels));
+}
+
+constexpr int kLowOut = 256;  // k written per block
+constexpr int kLowTop = 8;    // levels 0..kLowTop built in LDS
+constexpr int kLowSpan = kLowOut + (1 << kLowTop) - 1;
+constexpr int kLowThreads = 256;
+constexpr int kLowPer = (kLowSpan + kLowThreads - 1) / kLowThreads;
+
+// Levels 0..top (top <= kLowTop) for k in [k0, k0 + kLowOut).  Level d is
+// needed on [k0, k0 + kLowOut + 2^top - 2^d): each merge reads its own index
+// and the one 2^(d-1) to the right, so every entry of level d-1 is read
+// before any of level d is written (one barrier between the phases).
+__global__ __launch_bounds__(kLowThreads) void bla_low_kernel(BlaMut t,
+                                                              const double* __restrict__ zr,
+                                                              const double* __restrict__ zi,
+                                                              int type, int n, double kr,
+                                                              double ki, double tol, int top) {
+    __shared__ double4 sc[kLowSpan];
+    __shared__ double sr[kLowSpan];
+    const int k0 = blockIdx.x * kLowOut;
+    const int ref_len = t.ref_len;
+    const int span0 = kLowOut + (1 << top) - 1;
+    for (int i = threadIdx.x; i < span0; i += kLowThreads) {
+        const int k = k0 + i;
+        BlaEntry e = k < ref_len ? bla_level0(zr, zi, k, ref_len, type, n, kr, ki, tol) : bla_zero();
+        sc[i] = e.c;
+        sr[i] = e.r2;
+        if (i < kLowOut && k < ref_len) store_entry(t, k, 0, e);
+        if (i < kLowOut && k <= ref_len && (t.forms & AP_BLA_FORM_FP32))
+            t.zf[k] = make_float2((float)zr[k], (float)zi[k]);
+    }
+    for (int d = 1; d <= top; ++d) {
+        const int step = 1 << (d - 1);
+        const int span = kLowOut + (1 << top) - (1 << d);
+        BlaEntry v[kLowPer];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kLowPer; ++j) {
+            const int i = threadIdx.x + j * kLowThreads;
+            v[j] = bla_zero();
+            if (i < span && k0 + i < ref_len - 2 * step) {
+                BlaEntry e1, e2;
+                e1.c = sc[i];
+                e1.r2 = sr[i];
+                e2.c = sc[i + step];
+                e2.r2 = sr[i + step];
+                v[j] = bla_merge(e1, e2);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kLowPer; ++j) {
+            const int i = threadIdx.x + j * kLowThreads;
+            if (i < span) {
+                sc[i] = v[j].c;
+                sr[i] = v[j].r2;
+                if (i < kLowOut && k0 + i < ref_len) store_entry(t, k0 + i, d, v[j]);
+            }
         }
     }
 }
 
-__global__ __launch_bounds__(256) void bla_export_kernel(BlaView t, const double* __restrict__ r2m,
-                                                         double* __restrict__ out) {
+// Level d from level d-1 over all k: lanes k and k+1 read adjacent words.
+__global__ __launch_bounds__(256) void bla_merge_kernel(BlaMut t, int d) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= t.ref_len) return;
+    const size_t N = (size_t)t.ref_len;
+    const int step = 1 << (d - 1);
+    BlaEntry out = bla_zero();
+    if (k < t.ref_len - 2 * step) {
+        // All four loads issued together: one round trip to memory.
+        const size_t i1 = (size_t)(d - 1) * N + k, i2 = i1 + step;
+        BlaEntry e1, e2;
+        e1.r2 = t.r2[i1];
+        e2.r2 = t.r2[i2];
+        e1.c = t.coef[i1];
+        e2.c = t.coef[i2];
+        out = bla_merge(e1, e2);
+    }
+    store_entry(t, k, d, out);
+}
+
+__global__ __launch_bounds__(256) void bla_export_kernel(BlaView t, double* __restrict__ out) {
     size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;  // level-major index
     size_t n = (size_t)t.ref_len * t.levels;
     if (e >= n) return;
-    int d = (int)(e / t.ref_len), k = (int)(e % t.ref_len);
     double4 c = t.coef[e];
-    double* o = out + ((size_t)d * t.ref_len + k) * 5;
+    double* o = out + e * 5;
     o[0] = c.x;
     o[1] = c.y;
     o[2] = c.z;
     o[3] = c.w;
-    o[4] = r2m[e];
+    o[4] = t.r2[e];
 }
 
 size_t bla_bytes(int ref_len, int levels) { return bla_layout(ref_len, levels).total; }
 
 hipError_t launch_bla_build(const ap_orbit& orb, int type, int n, double kr, double ki,
-                            double tol, int levels, void* table, hipStream_t s) {
+                            double tol, int levels, int forms, void* table, hipStream_t s) {
     const int ref_len = orb.len;
-    BlaMut t = bla_mut(table, ref_len, levels);
-    int blocks = (ref_len + 255) / 256;
-    hipLaunchKernelGGL(bla_level0_kernel, dim3(blocks), dim3(256), 0, s, t, orb.re, orb.im, type,
-                       n, kr, ki, tol);
-    for (int d = 1; d < levels; ++d)
+    const int top = levels - 1 < kLowTop ? levels - 1 : kLowTop;
+    BlaMut t = bla_mut(table, ref_len, levels, forms, top);
+    // ceil((ref_len + 1) / kLowOut) blocks: the last one also covers Z_ref_len of zf.
+    hipLaunchKernelGGL(bla_low_kernel, dim3(ref_len / kLowOut + 1), dim3(kLowThreads), 0, s, t,
+                       orb.re, orb.im, type, n, kr, ki, tol, top);
+    const int blocks = (ref_len + 255) / 256;
+    for (int d = top + 1; d < levels; ++d)
         hipLaunchKernelGGL(bla_merge_kernel, dim3(blocks), dim3(256), 0, s, t, d);
-    hipLaunchKernelGGL(bla_derive_kernel, dim3((ref_len + kDeriveK - 1) / kDeriveK), dim3(256), 0,
-                       s, t);
     return hipGetLastError();
 }
 
 hipError_t launch_bla_export(const void* table, int ref_len, int levels, double* out,
                              hipStream_t s) {
     BlaView v = bla_view(table, ref_len, levels);
-    BlaLayout L = bla_layout(ref_len, levels);
-    const double* r2m = (const double*)((const char*)table + L.r2m);
     size_t ne = bla_entries(ref_len, levels);
     hipLaunchKernelGGL(bla_export_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, v,
-                       r2m, out);
+                       out);
     return hipGetLastError();
 }
 

@@ -4,24 +4,23 @@
 // Entry (k, d) advances the perturbation delta from iteration k by 2^d
 // iterations: dz' = A dz + B dc, valid while |dz| < r (reference
 // math_core/high_precision.py:1100-1238; the reference stores
-// float64[levels][ref_len][5]).  Here one allocation holds
-//   coef double4 [levels][ref_len]   (A_re, A_im, B_re, B_im), level-major:
-//                                    the merge of level d reads level d-1 at k
-//                                    and k + 2^(d-1) with coalesced loads
-//   r2m  double  [levels][ref_len]   r^2, level-major (build state)
-//   r2   double  [ref_len][levels]   r^2 as per-k rows (0 = invalid)
-//   r1   double  [ref_len][levels]   sqrt(r^2), the value the reference
-//                                    computes at lookup when |dz| is tiny
-//   c32  Coef32  [ref_len][levels]   the FP32-mantissa engine's entry: A and B
-//                                    as float32 mantissa pairs with an int32
-//                                    exponent each, and r^2 as an ordered
-//                                    int32 key (one 32-byte load per leap)
-// Lookups read one per-k row (the level search of one lane touches one row,
-// not `levels` scattered words).  The radii of one k never grow with d
-// (r_d = min(r_{d-1}, ...), and an invalid entry makes every higher level
-// invalid), so "the highest valid level" is found by binary search; it equals
-// the reference's top-down scan.  Entries past the end of the orbit
-// (k + 2^d >= ref_len) are invalid in the table itself.
+// float64[levels][ref_len][5]).  Here one allocation holds four level-major
+// regions, [levels][ref_len] each:
+//   coef double4   (A_re, A_im, B_re, B_im)
+//   r2   double    r^2 (0 = invalid)
+//   r1   double    sqrt(r^2), the value the reference computes at lookup
+//                  when |dz| is tiny
+//   c32  Coef32    the FP32-mantissa engine's entry: A and B as float32
+//                  mantissa pairs with an int32 exponent each, and r^2 as an
+//                  ordered int32 key (one 32-byte load per leap)
+// and, for the FP32 engine, the orbit Z_0..Z_ref_len rounded to float2 (zf).
+// Level-major keeps every build pass coalesced: the merge of level d reads
+// level d-1 at k and k + 2^(d-1) and writes level d at k, lanes on adjacent
+// k.  A lookup at one k reads one entry per probed level.  The radii of one
+// k never grow with d (r_d = min(r_{d-1}, ...), and an invalid entry makes
+// every higher level invalid), so "the highest valid level" is found by
+// binary search; it equals the reference's top-down scan.  Entries past the
+// end of the orbit (k + 2^d >= ref_len) are invalid in the table itself.
 #pragma once
 
 #include "ap_device.hpp"
@@ -34,8 +33,8 @@ namespace ap {
 struct __attribute__((aligned(32))) Coef32 {
     float ar, ai, br, bi;
     int ae, be;
-    int key;  // r2_key(r^2); kInvalidKey = invalid
-    int pad;
+    int key;      // r2_key(r^2); kInvalidKey = invalid
+    int next_hi;  // level_limit_by_length at k + 2^d (-1: no leap from there)
 };
 
 constexpr int kZeroExp = -(1 << 20);
@@ -58,16 +57,22 @@ __host__ __device__ inline int r2_key(double r2) {
 // fraction bits, the exponent field raised by 2 eff.  |eff| < 2^15 - 256.
 __device__ inline int dz_key(float q, int eff) { return (__float_as_int(q) >> 8) + (eff << 16); }
 
+// Element idx of a region, addressed by a 32-bit byte offset so the load
+// takes the region base from scalar registers.
+template <class T>
+__device__ inline const T& entry(const T* base, unsigned idx) {
+    return *(const T*)((const char*)base + idx * (unsigned)sizeof(T));
+}
+
 struct BlaView {
+    const double4* coef;
     const double* r2;
     const double* r1;
-    const double4* coef;
     const Coef32* c32;
-    int levels;   // 0 = no table
+    const float2* zf;
+    int levels;  // 0 = no table
     int ref_len;
-    // Tables stay below 2^31 entries (ap_bla_build checks), so 32-bit index
-    // arithmetic suffices.
-    __device__ double4 coef_at(int k, int d) const { return coef[(unsigned)(d * ref_len + k)]; }
+    __device__ unsigned at(int k, int d) const { return __umul24((unsigned)d, (unsigned)ref_len) + k; }
 };
 
 __host__ __device__ inline size_t bla_entries(int ref_len, int levels) {
@@ -78,39 +83,40 @@ __host__ __device__ inline size_t align256(size_t b) { return (b + 255) & ~(size
 
 // Byte offsets of the regions inside one table allocation.
 struct BlaLayout {
-    size_t coef, r2m, r2, r1, c32, total;
+    size_t coef, r2, r1, c32, zf, total;
 };
 
 __host__ __device__ inline BlaLayout bla_layout(int ref_len, int levels) {
     size_t n = bla_entries(ref_len, levels);
     BlaLayout L;
     L.coef = 0;
-    L.r2m = L.coef + align256(n * sizeof(double4));
-    L.r2 = L.r2m + align256(n * sizeof(double));
+    L.r2 = L.coef + align256(n * sizeof(double4));
     L.r1 = L.r2 + align256(n * sizeof(double));
     L.c32 = L.r1 + align256(n * sizeof(double));
-    L.total = L.c32 + align256(n * sizeof(Coef32));
+    L.zf = L.c32 + align256(n * sizeof(Coef32));
+    L.total = L.zf + align256((size_t)(ref_len > 0 ? ref_len + 1 : 0) * sizeof(float2));
     return L;
 }
 
 inline BlaView bla_view(const void* table, int ref_len, int levels) {
     BlaView b;
+    b.ref_len = ref_len;
     if (!table || levels < 2 || ref_len < 4) {
-        b.r2 = b.r1 = nullptr;
         b.coef = nullptr;
+        b.r2 = b.r1 = nullptr;
         b.c32 = nullptr;
+        b.zf = nullptr;
         b.levels = 0;
-        b.ref_len = ref_len;
         return b;
     }
     BlaLayout L = bla_layout(ref_len, levels);
     const char* p = (const char*)table;
+    b.coef = (const double4*)(p + L.coef);
     b.r2 = (const double*)(p + L.r2);
     b.r1 = (const double*)(p + L.r1);
-    b.coef = (const double4*)(p + L.coef);
     b.c32 = (const Coef32*)(p + L.c32);
+    b.zf = (const float2*)(p + L.zf);
     b.levels = levels;
-    b.ref_len = ref_len;
     return b;
 }
 
@@ -133,6 +139,12 @@ __device__ inline int highest_valid_level(int hi, Pred valid) {
 __device__ inline int level_limit_by_length(int k, int ref_len) {
     unsigned rem = (unsigned)(ref_len - k - 1);  // >= 1
     return 31 - __builtin_clz(rem);
+}
+
+// Highest level a leap from k may take: min(levels - 1, the length limit),
+// or -1 when no leap starts at k (k >= ref_len - 1).
+__device__ inline int top_level(int k, int ref_len, int levels) {
+    return k < ref_len - 1 ? min(levels - 1, level_limit_by_length(k, ref_len)) : -1;
 }
 
 }  // namespace ap

@@ -289,41 +289,100 @@ hipError_t launch_hist_cdf(const int* hist, int num_bins, float* cdf, hipStream_
 // -------------------------------------------------------------- colourise --
 // RGB from a cached float32 map: histogram CDF (scheme 11, apply_histogram_
 // equalization_kernel, math_core/cuda_kernels.py:2016-2035) or a palette
-// transfer function (recolor_iter_map_cuda_kernel, :3201-3222).  Four pixels
-// per lane: one 16-byte load of the map and 12 (or 24) contiguous RGB bytes.
-__global__ __launch_bounds__(256) void colorize_kernel(ap_view v, const float* __restrict__ it,
-                                                       const float* __restrict__ cdf,
-                                                       int num_bins,
-                                                       const float* __restrict__ lut,
-                                                       void* __restrict__ rgb, long long n) {
-    long long p0 = ((long long)blockIdx.x * 256 + threadIdx.x) * 4;
-    if (p0 >= n) return;
-    float s4[4];
-    if (p0 + 4 <= n) {
-        float4 q = *(const float4*)(it + p0);
-        s4[0] = q.x; s4[1] = q.y; s4[2] = q.z; s4[3] = q.w;
+// transfer function (recolor_iter_map_cuda_kernel, :3201-3222).  A block
+// takes 4096 pixels: each lane loads four 16-byte groups of the map (one
+// coalesced float4 per lane and round), and the block's RGB (12 or 24 KB) is
+// staged in LDS and leaves as contiguous 16-byte stores.  The last, partial
+// block stores per channel.
+constexpr int kColThreads = 256;
+constexpr int kColGroups = 4;  // float4 groups per lane
+constexpr int kColPx = 4 * kColGroups * kColThreads;
+
+// int(x + 0.5) truncates toward zero, as Python int() does.
+__device__ inline unsigned channel(double x) { return (unsigned)(int)(x + 0.5); }
+
+__device__ inline void colour_px(const ap_view& v, const ap_outputs& o, float s, unsigned& r,
+                                 unsigned& g, unsigned& b) {
+    double dr = 0.0, dg = 0.0, db = 0.0;
+    if (s >= 0.0f) sample_colour(v, o, s, dr, dg, db);
+    r = channel(dr);
+    g = channel(dg);
+    b = channel(db);
+}
+
+// RGB of the 4-pixel group `grp` of the block into its LDS slot: 3 packed
+// dwords for 8-bit channels, 6 for 16-bit.
+__device__ inline void colour_group(const ap_view& v, const ap_outputs& o, float4 q, int grp,
+                                    unsigned* st) {
+    unsigned r0, g0, b0, r1, g1, b1, r2, g2, b2, r3, g3, b3;
+    colour_px(v, o, q.x, r0, g0, b0);
+    colour_px(v, o, q.y, r1, g1, b1);
+    colour_px(v, o, q.z, r2, g2, b2);
+    colour_px(v, o, q.w, r3, g3, b3);
+    if (v.is_hdr) {
+        unsigned* w = st + 6 * grp;
+        w[0] = (r0 & 0xffffu) | (g0 << 16);
+        w[1] = (b0 & 0xffffu) | (r1 << 16);
+        w[2] = (g1 & 0xffffu) | (b1 << 16);
+        w[3] = (r2 & 0xffffu) | (g2 << 16);
+        w[4] = (b2 & 0xffffu) | (r3 << 16);
+        w[5] = (g3 & 0xffffu) | (b3 << 16);
     } else {
-        for (int t = 0; t < 4; ++t) s4[t] = p0 + t < n ? it[p0 + t] : -1.0f;
+        unsigned* w = st + 3 * grp;
+        w[0] = (r0 & 0xffu) | ((g0 & 0xffu) << 8) | ((b0 & 0xffu) << 16) | (r1 << 24);
+        w[1] = (g1 & 0xffu) | ((b1 & 0xffu) << 8) | ((r2 & 0xffu) << 16) | (g2 << 24);
+        w[2] = (b2 & 0xffu) | ((r3 & 0xffu) << 8) | ((g3 & 0xffu) << 16) | (b3 << 24);
     }
+}
+
+__global__ __launch_bounds__(kColThreads) void colorize_kernel(ap_view v, const float* __restrict__ it,
+                                                               const float* __restrict__ cdf,
+                                                               int num_bins,
+                                                               const float* __restrict__ lut,
+                                                               void* __restrict__ rgb, long long n) {
+    __shared__ uint4 stage[kColPx * 6 / 16];
+    const long long pb = (long long)blockIdx.x * kColPx;
     ap_outputs o = {};
     o.cdf = cdf;
     o.num_bins = num_bins;
     o.lut = lut;
-    for (int t = 0; t < 4 && p0 + t < n; ++t) {
-        double r = 0.0, g = 0.0, b = 0.0;
-        if (s4[t] >= 0.0f) sample_colour(v, o, s4[t], r, g, b);
-        store_rgb(rgb, v.is_hdr, p0 + t, r, g, b);
+    if (pb + kColPx > n) {
+        for (long long p = pb + threadIdx.x; p < n; p += kColThreads) {
+            const float s = it[p];
+            double r = 0.0, g = 0.0, b = 0.0;
+            if (s >= 0.0f) sample_colour(v, o, s, r, g, b);
+            store_rgb(rgb, v.is_hdr, p, r, g, b);
+        }
+        return;
     }
+    // Four loads in flight first, then four groups of work; named values
+    // (no per-lane arrays) keep everything in registers.
+    const float* src = it + pb + 4 * threadIdx.x;
+    const float4 q0 = *(const float4*)(src);
+    const float4 q1 = *(const float4*)(src + 4 * kColThreads);
+    const float4 q2 = *(const float4*)(src + 8 * kColThreads);
+    const float4 q3 = *(const float4*)(src + 12 * kColThreads);
+    unsigned* st = (unsigned*)stage;
+    // One copy of the group code (the palette math inlined four times, not
+    // sixteen), the group picked by selects rather than an indexed array.
+#pragma unroll 1
+    for (int j = 0; j < kColGroups; ++j) {
+        const float4 q = j == 0 ? q0 : (j == 1 ? q1 : (j == 2 ? q2 : q3));
+        colour_group(v, o, q, threadIdx.x + j * kColThreads, st);
+    }
+    __syncthreads();
+    const int words = v.is_hdr ? kColPx * 6 / 16 : kColPx * 3 / 16;
+    uint4* dst = (uint4*)((char*)rgb + pb * (v.is_hdr ? 6 : 3));
+    for (int i = threadIdx.x; i < words; i += kColThreads) dst[i] = stage[i];
 }
 
 hipError_t launch_colorize(const ap_view& v, const float* it, const float* cdf, int num_bins,
                            const float* lut, void* rgb, hipStream_t s) {
     long long n = (long long)v.width * v.height;
     if (n <= 0) return hipSuccess;
-    long long threads = (n + 3) / 4;
-    int blocks = (int)((threads + 255) / 256);
-    hipLaunchKernelGGL(colorize_kernel, dim3(blocks), dim3(256), 0, s, v, it, cdf, num_bins, lut,
-                       rgb, n);
+    int blocks = (int)((n + kColPx - 1) / kColPx);
+    hipLaunchKernelGGL(colorize_kernel, dim3(blocks), dim3(kColThreads), 0, s, v, it, cdf, num_bins,
+                       lut, rgb, n);
     return hipGetLastError();
 }
 

@@ -17,10 +17,13 @@ hipError_t launch_pert64(const ap_view& v, const ap_orbit& orb, const void* bla,
 hipError_t launch_fexp(int mant, const ap_view& v, const ap_orbit& orb, const void* bla,
                        const ap_outputs& o, void* scratch_f32_orbit, hipStream_t s);
 
-// BLA table.
+// BLA table.  Limits: ref_len < 2^24 (24-bit index multiplies) and at most
+// 2^27 entries (32-bit byte offsets into the 32-byte region).
+constexpr int kMaxBlaRefLen = (1 << 24) - 1;
+constexpr long long kMaxBlaEntries = 1LL << 27;
 size_t bla_bytes(int ref_len, int levels);
 hipError_t launch_bla_build(const ap_orbit& orb, int type, int n, double kr, double ki,
-                            double tol, int levels, void* table, hipStream_t s);
+                            double tol, int levels, int forms, void* table, hipStream_t s);
 hipError_t launch_bla_export(const void* table, int ref_len, int levels, double* out,
                              hipStream_t s);
 

@@ -89,6 +89,15 @@ __device__ inline Sample nan_exit(int k) {
     return r;
 }
 
+// Step 0 of a Mandelbrot-type orbit (types 0 and 4, Z_0 = 0, dz_0 = 0) gives
+// dz_1 = dc exactly: every engine's step formula reduces to it, and neither
+// the escape test (|Z_0|^2 = 0) nor the glitch test (k > 15) fires at k = 0.
+// The engines start such pixels at k = 1; the skipped step counts as plain.
+template <int T>
+__device__ inline bool mandel_start(const ap_view& v, const double* zr, const double* zi) {
+    return (T == 0 || T == 4) && v.ref_len >= 1 && v.max_iter >= 1 && zr[0] == 0.0 && zi[0] == 0.0;
+}
+
 struct PertConsts {
     double inv_ref, inv_exact;
 };
@@ -175,6 +184,11 @@ __device__ Sample pert64_sample(const ap_view& v, const PertData& D, double si, 
     int k = 0, glitch = 0, plain = 0, nbla = 0;
     const int n = v.gen_n;
     const double kr = v.gen_kr, ki = v.gen_ki;
+    if (mandel_start<T>(v, D.zr, D.zi)) {
+        dzx = dpx;
+        dzy = dpy;
+        k = plain = 1;
+    }
 
     while (k < ref_len) {
         const double rx = D.zr[k], ry = D.zi[k];
@@ -194,7 +208,6 @@ __device__ Sample pert64_sample(const ap_view& v, const PertData& D, double si, 
         }
         if (has_bla && k < ref_len - 1) {
             const double dz_mag2 = dzx * dzx + dzy * dzy;
-            const unsigned row = (unsigned)(k * D.bla.levels);
             int max_d = D.bla.levels - 1;
             if (dz_mag2 == 0.0 && (dzx != 0.0 || dzy != 0.0)) max_d = max_d < 8 ? max_d : 8;
             const int hi = min(max_d, level_limit_by_length(k, ref_len));
@@ -203,14 +216,18 @@ __device__ Sample pert64_sample(const ap_view& v, const PertData& D, double si, 
             // The top level's coefficients are fetched with the radii: in deep
             // views the top level is the one taken, and the leap then costs a
             // single round trip to the cache.
-            const double r0 = D.bla.r2[row], rtop = D.bla.r2[row + hi];
-            const double4 ctop = D.bla.coef_at(k, hi);
+            const unsigned itop = D.bla.at(k, hi);
+            const double r0 = entry(D.bla.r2, (unsigned)k), rtop = entry(D.bla.r2, itop);
+            const double4 ctop = entry(D.bla.coef, itop);
             if (dz_mag2 < r0) {
                 int best = dz_mag2 < rtop ? hi
                                           : highest_valid_level(hi - 1, [&](int d) {
-                                                return dz_mag2 < D.bla.r2[row + d];
+                                                return dz_mag2 < entry(D.bla.r2, D.bla.at(k, d));
                                             });
-                double4 c = best == hi ? ctop : D.bla.coef_at(k, best);
+                // A value copy, not a conditional lvalue: selecting between a
+                // local and a table reference would put the local in scratch.
+                double4 c = ctop;
+                if (best != hi) c = entry(D.bla.coef, D.bla.at(k, best));
                 double ndx = (c.x * dzx - c.y * dzy) + (c.z * dpx - c.w * dpy);
                 dzy = (c.x * dzy + c.y * dzx) + (c.z * dpy + c.w * dpx);
                 dzx = ndx;
@@ -283,36 +300,38 @@ struct EngPert64 {
 __device__ inline int bla_leap64(const PertData& D, double& dzx, double& dzy, int& z_exp,
                                  double dpx, double dpy, int k, int E, int ref_len) {
     const int shift = z_exp - E;
-    const unsigned row = (unsigned)(k * D.bla.levels);
     const int hi = min(D.bla.levels - 1, level_limit_by_length(k, ref_len));
+    const unsigned i0 = (unsigned)k, itop = D.bla.at(k, hi);
     bool can0, cantop;
     double dz_mag2 = 0.0, tiny = 0.0;
     if (shift >= -510) {
         double sc = pow2_f64(shift);
         dz_mag2 = (dzx * dzx + dzy * dzy) * (sc * sc);
-        const double r0 = D.bla.r2[row], rtop = D.bla.r2[row + hi];
+        const double r0 = entry(D.bla.r2, i0), rtop = entry(D.bla.r2, itop);
         can0 = dz_mag2 < r0;
         cantop = dz_mag2 < rtop;
     } else {
         double dz_mag = sqrt(dzx * dzx + dzy * dzy);
         tiny = dz_mag * pow2_f64(shift > -1020 ? shift : -1020);
-        const double r0 = D.bla.r2[row], rtop = D.bla.r2[row + hi];
-        const double s0 = D.bla.r1[row], stop = D.bla.r1[row + hi];
+        const double r0 = entry(D.bla.r2, i0), rtop = entry(D.bla.r2, itop);
+        const double s0 = entry(D.bla.r1, i0), stop = entry(D.bla.r1, itop);
         can0 = r0 > 0.0 && tiny < s0;
         cantop = rtop > 0.0 && tiny < stop;
     }
-    const double4 ctop = D.bla.coef_at(k, hi);
+    const double4 ctop = entry(D.bla.coef, itop);
     if (!can0) return 0;
     int best;
     if (cantop)
         best = hi;
     else if (shift >= -510)
-        best = highest_valid_level(hi - 1, [&](int d) { return dz_mag2 < D.bla.r2[row + d]; });
+        best = highest_valid_level(hi - 1, [&](int d) { return dz_mag2 < entry(D.bla.r2, D.bla.at(k, d)); });
     else
         best = highest_valid_level(hi - 1, [&](int d) {
-            return D.bla.r2[row + d] > 0.0 && tiny < D.bla.r1[row + d];
+            const unsigned i = D.bla.at(k, d);
+            return entry(D.bla.r2, i) > 0.0 && tiny < entry(D.bla.r1, i);
         });
-    double4 c = best == hi ? ctop : D.bla.coef_at(k, best);
+    double4 c = ctop;  // value copy (a conditional lvalue would go to scratch)
+    if (best != hi) c = entry(D.bla.coef, D.bla.at(k, best));
     double scale_c = (z_exp >= -1020 && z_exp <= 1020) ? pow2_f64(-z_exp) : 0.0;
     double dxs = dpx * scale_c, dys = dpy * scale_c;
     double ndx = (c.x * dzx - c.y * dzy) + (c.z * dxs - c.w * dys);
@@ -337,15 +356,25 @@ __device__ Sample fexp64_sample(const ap_view& v, const PertData& D, double si, 
     int k = 0, glitch = 0, plain = 0, nbla = 0;
     const int n = v.gen_n;
     const double kr = v.gen_kr, ki = v.gen_ki;
+    if (mandel_start<T>(v, D.zr, D.zi)) {
+        dzx = dpx;
+        dzy = dpy;
+        renorm64(dzx, dzy, z_exp);
+        k = plain = 1;
+    }
 
     while (k < ref_len) {
         // Deep regime (eff < -50): the reference's escape test (eff >= -50) and
         // the Pauldelbrot test (returns 0 below eff = -40) are both skipped, so
         // consecutive BLA leaps run in a loop of their own.
+        bool no_leap = false;  // level 0 already failed at this k and state
         if ((T == 0 || T == 2 || T == 4) && has_bla) {
             while (z_exp - E < -50 && k < ref_len - 1) {
                 int step = bla_leap64(D, dzx, dzy, z_exp, dpx, dpy, k, E, ref_len);
-                if (step == 0) break;
+                if (step == 0) {
+                    no_leap = true;
+                    break;
+                }
                 ++nbla;
                 if (isnan(dzx) || isnan(dzy) || isinf(dzx) || isinf(dzy)) {
                     Sample r = nan_exit(k);
@@ -378,7 +407,7 @@ __device__ Sample fexp64_sample(const ap_view& v, const PertData& D, double si, 
             if (glitch_floatexp(rx, ry, dzx, dzy, eff)) glitch = 1;
         }
         if (T == 0 || T == 2 || T == 4) {
-            if (has_bla && k < ref_len - 1) {
+            if (has_bla && !no_leap && k < ref_len - 1) {
                 int step = bla_leap64(D, dzx, dzy, z_exp, dpx, dpy, k, E, ref_len);
                 if (step > 0) {
                     ++nbla;
@@ -521,56 +550,106 @@ __device__ inline void renorm32(float& dzx, float& dzy, int& z_exp) {
     }
 }
 
-// Row entry idx of the FP32 table.  The 32-bit byte offset (the table holds
-// at most 2^27 entries) lets the load take the table base from scalar
-// registers.
-__device__ inline const Coef32& c32_entry(const Coef32* base, unsigned idx) {
-    return *(const Coef32*)((const char*)base + (idx << 5));
+// BLA leaps of the FP32 engine.  The highest level a leap from k may take is
+// hi = top_level(k); the entry at level hi is fetched together with the key
+// test, and in deep views it is the one taken, so a leap costs one round trip
+// to the cache.  When it is not valid, leap32_search finds the highest valid
+// level below it (-1 when level 0 is not valid either).
+//
+// dz' = A dz + B dc with dz = m 2^z_exp, A = am 2^ae, B = bm 2^be and dc in
+// units of 2^-E (dc_m = dc 2^E, of order one): the A term has exponent
+// e1 = z_exp + ae + em, 2^em a bound on |m| read off the key's |m|^2, the B
+// term e2 = be.  Both inputs are scaled down to ne = max(e1, e2) before the
+// products, which run as FMAs; the result, at exponent ne, is bounded by a
+// few units (|am| < 2 sqrt 2, scaled |m| < sqrt 2, plus |bm| |dc_m|), so no
+// normalisation step follows.  kNoDc: the B dc term is zero (Julia sets), so
+// its exponent takes no part.
+__device__ inline int leap32_search(const Fexp32Data& D, int k, int hi, int key) {
+    if (!(key < entry(D.bla.c32, (unsigned)k).key)) return -1;
+    return highest_valid_level(hi - 1, [&](int m) { return key < entry(D.bla.c32, D.bla.at(k, m)).key; });
 }
 
-// One BLA leap of the FP32 engine from iteration k: returns the number of
-// iterations advanced (0 = level 0 not valid).  The entry of the highest
-// level the orbit length allows is fetched together with the key test; in
-// deep views it is the one taken, and the leap costs one round trip to the
-// cache.  dz' = A dz + B dc with A = am 2^ae and B = bm 2^be: the terms have
-// exponents e1 = z_exp + ae and e2 = be (dc in units of 2^-E).  Both inputs
-// are scaled to the larger exponent before the products, which then run as
-// FMAs, and the result is normalised to max|dz_m| in [1/2, 1).  kNoDc: the
-// B dc term is zero (Julia sets), so its exponent takes no part.
 template <bool kNoDc>
-__device__ inline int bla_leap32(const Fexp32Data& D, float& dzx, float& dzy, int& z_exp,
-                                 float dpx, float dpy, int k, int E, int ref_len) {
-    const int L = D.bla.levels;
-    const unsigned row = __umul24(k, L);  // k < 2^24 (table bound 2^27 entries)
-    const int hi = min(L - 1, level_limit_by_length(k, ref_len));
-    const Coef32 top = c32_entry(D.bla.c32, row + hi);
-    const int key = dz_key(fmaf(dzx, dzx, dzy * dzy), z_exp - E);
-    int d = hi;
-    if (!(key < top.key)) {
-        if (!(key < c32_entry(D.bla.c32, row).key)) return 0;
-        d = highest_valid_level(hi - 1, [&](int m) { return key < c32_entry(D.bla.c32, row + m).key; });
-    }
-    const Coef32 c = d == hi ? top : c32_entry(D.bla.c32, row + d);
-    const int e1 = z_exp + c.ae;
-    float rx, ry;
-    int ne;
+__device__ inline void leap32_terms(const Coef32& c, float q, float dzx, float dzy, int z_exp,
+                                    float dpx, float dpy, float& rx, float& ry, int& ne) {
+    // |m| < 2^em: q = |m|^2 has biased exponent field b, so |m|^2 < 2^(b - 126)
+    // and em = (b >> 1) - 62 bounds it (q = 0 gives a small em).
+    const int em = (int)(__float_as_uint(q) >> 24) - 62;
+    const int za = z_exp + c.ae;
     if (kNoDc) {
-        ne = e1;
-        rx = fmaf(c.ar, dzx, -c.ai * dzy);
-        ry = fmaf(c.ar, dzy, c.ai * dzx);
+        ne = za + em;
+        const float mx = ldexpf(dzx, -em), my = ldexpf(dzy, -em);
+        rx = fmaf(c.ar, mx, -c.ai * my);
+        ry = fmaf(c.ar, my, c.ai * mx);
     } else {
-        const int e2 = c.be;
-        ne = max(e1, e2);
-        const float mx = ldexpf(dzx, e1 - ne), my = ldexpf(dzy, e1 - ne);
-        const float px = ldexpf(dpx, e2 - ne), py = ldexpf(dpy, e2 - ne);
+        ne = max(za + em, c.be);
+        const float mx = ldexpf(dzx, za - ne), my = ldexpf(dzy, za - ne);
+        const float px = ldexpf(dpx, c.be - ne), py = ldexpf(dpy, c.be - ne);
         rx = fmaf(c.ar, mx, fmaf(-c.ai, my, fmaf(c.br, px, -c.bi * py)));
         ry = fmaf(c.ar, my, fmaf(c.ai, mx, fmaf(c.br, py, c.bi * px)));
     }
-    const int e = __builtin_amdgcn_frexp_expf(fmaxf(fabsf(rx), fabsf(ry)));
-    dzx = ldexpf(rx, -e);
-    dzy = ldexpf(ry, -e);
-    z_exp = ne + e;
-    return 1 << d;
+}
+
+// One leap from k if level 0 is valid: returns the level taken or -1.
+template <bool kNoDc>
+__device__ inline int bla_leap32(const Fexp32Data& D, float& dzx, float& dzy, int& z_exp,
+                                 float dpx, float dpy, int k, int hi, int E) {
+    Coef32 c = entry(D.bla.c32, D.bla.at(k, hi));
+    const float q = fmaf(dzx, dzx, dzy * dzy);
+    const int key = dz_key(q, z_exp - E);
+    int d = hi;
+    if (!(key < c.key)) {
+        d = leap32_search(D, k, hi, key);
+        if (d < 0) return -1;
+        c = entry(D.bla.c32, D.bla.at(k, d));
+    }
+    float rx, ry;
+    int ne;
+    leap32_terms<kNoDc>(c, q, dzx, dzy, z_exp, dpx, dpy, rx, ry, ne);
+    dzx = rx;
+    dzy = ry;
+    z_exp = ne;
+    return d;
+}
+
+// Consecutive leaps while z_exp < z_deep, in a loop with a single exit and
+// a single (rare) branch, so the exec-mask bookkeeping per leap stays small:
+// a lane whose level 0 is not valid at its k applies the identity entry
+// (A = 1, B = 0, which rescales its mantissa exactly) with next_hi = -2, and
+// leaves the loop; the -2 then sets no_leap.
+template <bool kNoDc>
+__device__ inline void deep_leaps32(const Fexp32Data& D, float& dzx, float& dzy, int& z_exp,
+                                    float dpx, float dpy, int& k, int& nbla, bool& no_leap,
+                                    int E, int z_deep, int ref_len) {
+    int hi = top_level(k, ref_len, D.bla.levels);
+    while (z_exp < z_deep && hi >= 0) {
+        Coef32 c = entry(D.bla.c32, D.bla.at(k, hi));
+        const float q = fmaf(dzx, dzx, dzy * dzy);
+        const int key = dz_key(q, z_exp - E);
+        int d = hi;
+        if (!(key < c.key)) {
+            d = leap32_search(D, k, hi, key);
+            if (d >= 0) {
+                c = entry(D.bla.c32, D.bla.at(k, d));
+            } else {
+                c.ar = 1.0f;
+                c.ai = c.br = c.bi = 0.0f;
+                c.ae = 0;
+                c.be = kZeroExp;
+                c.next_hi = -2;
+            }
+        }
+        float rx, ry;
+        int ne;
+        leap32_terms<kNoDc>(c, q, dzx, dzy, z_exp, dpx, dpy, rx, ry, ne);
+        dzx = rx;
+        dzy = ry;
+        z_exp = ne;
+        k += d >= 0 ? 1 << d : 0;
+        nbla += d >= 0 ? 1 : 0;
+        hi = c.next_hi;
+    }
+    no_leap = hi == -2;
 }
 
 // Hand-off point of the FP32 loop: once |dz| >= 2^kTail32 the delta continues
@@ -677,20 +756,23 @@ __device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si
     int k = 0, glitch = 0, plain = 0, nbla = 0;
     const int n = v.gen_n;
     const float kr = (float)v.gen_kr, ki = (float)v.gen_ki;
+    const int z_deep = E + kTail32 - 40;
+    if (mandel_start<T>(v, D.zr, D.zi)) {
+        dzx = dpx;
+        dzy = dpy;
+        renorm32(dzx, dzy, z_exp);
+        k = plain = 1;
+    }
 
     while (k < ref_len) {
-        // Deep regime: max|dz_m| < 2^32, so |dz| < 2^(eff + 32) < 2^(kTail32 - 8):
-        // no hand-off, no glitch test (it needs eff >= -40) and no escape can
-        // happen; only BLA leaps run, in a loop of their own.  A leap of
-        // finite mantissas (|dz_m| < 2^32, |am|, |bm| < 2, shifts <= 0) is
-        // finite, so the loop has no NaN test.
+        // Deep regime (z_exp < z_deep, i.e. eff < kTail32 - 40): max|dz_m| < 2^32,
+        // so |dz| < 2^(eff + 32) < 2^(kTail32 - 8): no hand-off, no glitch test
+        // (it needs eff >= -40) and no escape can happen; only BLA leaps run,
+        // in a loop of their own.  A leap of finite mantissas (inputs scaled
+        // to at most 1, |am|, |bm| < 2) is finite, so the loop has no NaN test.
+        bool no_leap = false;  // level 0 already failed at this k and state
         if ((T == 0 || T == 2 || T == 4) && has_bla) {
-            while (z_exp - E < kTail32 - 40 && k < ref_len - 1) {
-                int step = bla_leap32<julia>(D, dzx, dzy, z_exp, dpx, dpy, k, E, ref_len);
-                if (step == 0) break;
-                ++nbla;
-                k += step;
-            }
+            deep_leaps32<julia>(D, dzx, dzy, z_exp, dpx, dpy, k, nbla, no_leap, E, z_deep, ref_len);
             if (k >= ref_len) break;
         }
         const int eff = z_exp - E;
@@ -713,11 +795,12 @@ __device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si
             }
         }
         if (T == 0 || T == 2 || T == 4) {
-            if (has_bla && k < ref_len - 1) {
-                int step = bla_leap32<julia>(D, dzx, dzy, z_exp, dpx, dpy, k, E, ref_len);
-                if (step > 0) {
+            if (has_bla && !no_leap && k < ref_len - 1) {
+                const int d = bla_leap32<julia>(D, dzx, dzy, z_exp, dpx, dpy, k,
+                                                top_level(k, ref_len, D.bla.levels), E);
+                if (d >= 0) {
                     ++nbla;
-                    k += step;
+                    k += 1 << d;
                     continue;
                 }
             }
@@ -819,11 +902,15 @@ hipError_t launch_fexp(int mant, const ap_view& v, const ap_orbit& orb, const vo
     Fexp32Data d;
     d.zr = orb.re;
     d.zi = orb.im;
-    d.zf = (const float2*)scratch_f32_orbit;
     d.bla = b;
-    int n = orb.len + 1;
-    hipLaunchKernelGGL(orbit_to_f32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, orb.re, orb.im,
-                       n, (float2*)scratch_f32_orbit);
+    if (b.levels > 0) {
+        d.zf = b.zf;  // written with the table's FP32 form
+    } else {
+        d.zf = (const float2*)scratch_f32_orbit;
+        int n = orb.len + 1;
+        hipLaunchKernelGGL(orbit_to_f32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, orb.re,
+                           orb.im, n, (float2*)scratch_f32_orbit);
+    }
     return dispatch_pert<EngFexp32>(v, o, d, s);
 }
 

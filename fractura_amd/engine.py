@@ -198,10 +198,15 @@ class Frame:
                 "samples": int(s[3])}
 
 
+def bla_forms(engine):
+    """The BLA lookup forms the given engine's pass reads."""
+    return _abi.AP_BLA_FORM_FP32 if engine == "fexp32" else _abi.AP_BLA_FORM_FP64
+
+
 class _Orbit:
     """Device copy of a reference orbit and its BLA table."""
 
-    def __init__(self, dev, plan, slot, tag="ref"):
+    def __init__(self, dev, plan, slot, tag="ref", forms=None):
         torch = __import__("torch")
         re, im, ref_len = plan.orbit
         self.len = ref_len
@@ -218,9 +223,12 @@ class _Orbit:
             levels = BLA_LEVELS
             nbytes = dev.lib.ap_bla_bytes(ref_len, levels)
             self.table = get_scratch_bytes(nbytes, f"{tag}_bla_{slot}", dev.index)
+            if forms is None:
+                forms = bla_forms(plan.engine)
             dev.call("ap_bla_build", C.byref(self.abi), v.fractal_type, v.gen_n, v.gen_kr, v.gen_ki,
-                     plan.bla_tol, levels, ptr(self.table))
+                     plan.bla_tol, levels, forms, ptr(self.table))
             self.levels = levels
+            self.forms = forms
 
     def table_ptr(self):
         return ptr(self.table) if self.table is not None else None
@@ -236,6 +244,36 @@ def _launch(dev, engine, view, orbit, outs):
         fn = {"pert64": "ap_render_pert64", "fexp64": "ap_render_fexp64",
               "fexp32": "ap_render_fexp32"}[engine]
         dev.call(fn, C.byref(view), C.byref(orbit.abi), orbit.table_ptr(), C.byref(outs))
+
+
+def capture_graph(dev, fn, warmup=1):
+    """Capture the library launches `fn` makes (BLA build, pixel pass,
+    colouring, ...) into a HIP graph for replay with torch.cuda.CUDAGraph.
+
+    `fn` must enqueue device work only: no host synchronisation, no device
+    allocation (every buffer it uses is allocated beforehand and stays alive,
+    at the same address, while the graph is replayed), and no memset of the
+    library (it records none on the paths a frame takes).  One warm-up call
+    runs eagerly on a side stream first.  Streams keep the default priority.
+    """
+    import torch
+
+    cur = torch.cuda.current_stream(dev.index)
+    side = torch.cuda.Stream(device=dev.torch_device)
+    side.wait_stream(cur)
+    with torch.cuda.stream(side):
+        for _ in range(warmup):
+            dev.set_stream(side)
+            fn()
+    cur.wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    try:
+        with torch.cuda.graph(g):
+            dev.set_stream(None)  # torch's capture stream is current here
+            fn()
+    finally:
+        dev.set_stream(None)
+    return g
 
 
 def _outs(counts=None, smooth=None, iter_map=None, glitch=None, rgb=None, lut=None, mask=None,

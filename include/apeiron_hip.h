@@ -105,11 +105,13 @@ typedef struct ap_outputs {
                              -1 where the cardioid/bulb shortcut applied     */
     double *smooth;       /* float64 smooth value k+1-log2(ln|z|/ln 256);
                              -1 for the interior                             */
-    float *iter_map;      /* reference float32 smooth map (-1 interior)      */
+    float *iter_map;      /* reference float32 smooth map (-1 interior);
+                             16-byte aligned                                  */
     uint8_t *glitch;      /* glitch flags (perturbation engines)             */
-    void *rgb;            /* (H, W, 3) uint8, or uint16 if view.is_hdr; a
-                             centre pass (mode 0) colours iter_map after the
-                             iteration kernel, so it needs iter_map too       */
+    void *rgb;            /* (H, W, 3) uint8, or uint16 if view.is_hdr,
+                             16-byte aligned; a centre pass (mode 0) colours
+                             iter_map after the iteration kernel, so it needs
+                             iter_map too                                    */
     const float *lut;     /* (2048, 3) float32 colour LUT                    */
     const uint8_t *mask;  /* edge mask (mode 2) or glitch-in mask (rebase)   */
     const float *cdf;     /* histogram CDF: colour subsamples by scheme 11   */
@@ -176,14 +178,23 @@ int ap_render_fexp32(ap_ctx *ctx, const ap_view *view, const ap_orbit *orbit,
 /* ---- chained BLA table -----------------------------------------------------
  * Replaces _jit_build_bla_table (math_core/high_precision.py:1100-1238) and
  * upload_bla_table_cuda (math_core/engine.py:416-423): the table is built on
- * the device, one launch per dyadic level.  ap_bla_bytes gives the size of
- * the caller-allocated device buffer; the layout is this library's own.
+ * the device: levels 0..8 by one kernel in LDS, then one launch per higher
+ * dyadic level.  ap_bla_bytes gives the size of the caller-allocated device
+ * buffer; the layout is this library's own.
  * ap_bla_export writes the reference layout float64[levels][ref_len][5]
  * (A_re, A_im, B_re, B_im, r^2) into a device buffer, for comparison. */
 size_t ap_bla_bytes(int32_t ref_len, int32_t levels);
+/* `forms` (bit mask) names the lookup forms to write; a pass must not be
+ * given a table built without the form it reads.  The FP64 form (A, B, r^2,
+ * r) serves ap_render_pert64, ap_render_fexp64 and ap_bla_export; the FP32
+ * form (float32 mantissas, int32 exponents, ordered r^2 key) serves
+ * ap_render_fexp32. */
+#define AP_BLA_FORM_FP64 1
+#define AP_BLA_FORM_FP32 2
+#define AP_BLA_FORM_ALL 3
 int ap_bla_build(ap_ctx *ctx, const ap_orbit *orbit, int32_t fractal_type,
                  int32_t gen_n, double gen_kr, double gen_ki, double tol,
-                 int32_t levels, void *table);
+                 int32_t levels, int32_t forms, void *table);
 int ap_bla_export(ap_ctx *ctx, const void *table, int32_t ref_len,
                   int32_t levels, double *out_ref_layout);
 
@@ -207,7 +218,7 @@ int ap_glitch_bins(ap_ctx *ctx, const uint8_t *mask, int32_t width,
  * scan_blocks / scan_single_block / finalize_normalized_cdf kernels
  * (:1929-1996); ap_colorize replaces apply_histogram_equalization_kernel
  * (:2016-2035) for scheme 11 and recolor_iter_map_cuda_kernel (:3201-3222)
- * for schemes 0..10. */
+ * for schemes 0..10; iter_map and rgb must be 16-byte aligned. */
 int ap_hist_build(ap_ctx *ctx, const float *iter_map, int32_t n,
                   int32_t max_iter, int32_t num_bins, int32_t *hist);
 int ap_hist_cdf(ap_ctx *ctx, const int32_t *hist, int32_t num_bins, float *cdf);

@@ -701,7 +701,8 @@ Res fexp32_sample(const ap_view& v, const Orbit32& O, double si, double sj) {
         if (has_bla && k < ref_len - 1) {
             // Highest valid level by a top-down scan of the row (the device
             // binary-searches the same monotone keys).
-            const int key = dz_key(fmaf(dzx, dzx, dzy * dzy), ze - E);
+            const float q = fmaf(dzx, dzx, dzy * dzy);
+            const int key = dz_key(q, ze - E);
             int best = -1;
             for (int d = L - 1; d >= 0; --d)
                 if (k + (1 << d) < ref_len && key < O.c32[(size_t)k * L + d].key) {
@@ -709,27 +710,30 @@ Res fexp32_sample(const ap_view& v, const Orbit32& O, double si, double sj) {
                     break;
                 }
             if (best >= 0) {
+                // Both terms scaled to the larger exponent, FMA products, no
+                // normalisation of the result.
                 const Coef32 c = O.c32[(size_t)k * L + best];
-                const int e1 = ze + c.ae;
+                uint32_t qb;
+                memcpy(&qb, &q, 4);
+                const int em = (int)(qb >> 24) - 62;  // |m| < 2^em
+                const int za = ze + c.ae;
                 float rx, ry;
                 int ne;
                 if (julia) {
-                    ne = e1;
-                    rx = fmaf(c.ar, dzx, -c.ai * dzy);
-                    ry = fmaf(c.ar, dzy, c.ai * dzx);
+                    ne = za + em;
+                    const float mx = ldexpf(dzx, -em), my = ldexpf(dzy, -em);
+                    rx = fmaf(c.ar, mx, -c.ai * my);
+                    ry = fmaf(c.ar, my, c.ai * mx);
                 } else {
-                    const int e2 = c.be;
-                    ne = std::max(e1, e2);
-                    const float mx = ldexpf(dzx, e1 - ne), my = ldexpf(dzy, e1 - ne);
-                    const float px = ldexpf(dpx, e2 - ne), py = ldexpf(dpy, e2 - ne);
+                    ne = std::max(za + em, c.be);
+                    const float mx = ldexpf(dzx, za - ne), my = ldexpf(dzy, za - ne);
+                    const float px = ldexpf(dpx, c.be - ne), py = ldexpf(dpy, c.be - ne);
                     rx = fmaf(c.ar, mx, fmaf(-c.ai, my, fmaf(c.br, px, -c.bi * py)));
                     ry = fmaf(c.ar, my, fmaf(c.ai, mx, fmaf(c.br, py, c.bi * px)));
                 }
-                int e;
-                frexpf(std::max(fabsf(rx), fabsf(ry)), &e);
-                dzx = ldexpf(rx, -e);
-                dzy = ldexpf(ry, -e);
-                ze = ne + e;
+                dzx = rx;
+                dzy = ry;
+                ze = ne;
                 k += 1 << best;
                 continue;
             }

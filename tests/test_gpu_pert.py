@@ -61,7 +61,7 @@ def test_bla_build_bit_exact(ftype, cx, cy, n):
     re, im, ref_len = plan.orbit
     dev = engine.get_device(0)
     dev.set_stream(None)
-    orbit = engine._Orbit(dev, plan, 0)
+    orbit = engine._Orbit(dev, plan, 0, forms=_abi.AP_BLA_FORM_ALL)
     out = torch.empty((16, ref_len, 5), dtype=torch.float64, device=dev.torch_device)
     dev.call("ap_bla_export", engine.ptr(orbit.table), ref_len, 16, engine.ptr(out))
     _sync()
@@ -75,6 +75,32 @@ def test_bla_build_bit_exact(ftype, cx, cy, n):
         # a few ulps; the level-0 coefficients stay exact.
         assert np.array_equal(got[0, :, :4].view(np.uint64), ref[0, :, :4].view(np.uint64))
         assert np.allclose(got, ref, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("engine_name", ["fexp32", "fexp64"])
+def test_bla_forms_render_like_a_full_table(engine_name):
+    """A table built with only the form a pass reads (the merges then keep A,
+    B, r^2 only for the levels they read back) renders exactly like a table
+    built with every form."""
+    import torch
+
+    plan = _plan("0", "1", "1e-320", W=64, H=36, max_iter=6000,
+                 fexp_mantissa=32 if engine_name == "fexp32" else 64)
+    assert plan.engine == engine_name and plan.bla
+    dev = engine.get_device(0)
+    dev.set_stream(None)
+    outs = {}
+    for forms in (_abi.AP_BLA_FORM_ALL, engine.bla_forms(engine_name)):
+        orbit = engine._Orbit(dev, plan, 0, forms=forms)
+        counts = torch.empty((36, 64), dtype=torch.int32, device=dev.torch_device)
+        smooth = torch.empty((36, 64), dtype=torch.float64, device=dev.torch_device)
+        engine._launch(dev, engine_name, plan.view.copy(), orbit, engine._outs(counts, smooth))
+        _sync()
+        outs[forms] = (to_np(counts), to_np(smooth))
+    (c_all, s_all), (c_one, s_one) = outs.values()
+    assert np.array_equal(c_all, c_one)
+    assert np.array_equal(s_all.view(np.uint64), s_one.view(np.uint64))
+    assert np.mean(c_all < plan.view.max_iter) > 0.5  # the pixels escape after BLA leaps
 
 
 def test_pert64_seahorse_matches_oracle():
