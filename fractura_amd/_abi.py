@@ -72,6 +72,14 @@ class ApOrbit(C.Structure):
     _fields_ = [("re", C.c_void_p), ("im", C.c_void_p), ("len", C.c_int32), ("reserved0", C.c_int32)]
 
 
+class ApFrame(C.Structure):
+    _fields_ = [
+        ("view", ApView), ("orbit", ApOrbit), ("bla", C.c_void_p), ("out", ApOutputs),
+        ("tile_begin", C.c_int32), ("tiles_x", C.c_int32), ("tiles", C.c_int32),
+        ("reserved0", C.c_int32),
+    ]
+
+
 _P = C.c_void_p
 _I = C.c_int32
 _D = C.c_double
@@ -106,6 +114,8 @@ SIGNATURES = {
     "ap_hist_build": (C.c_int, [_P, _P, _I, _I, _I, _P]),
     "ap_hist_cdf": (C.c_int, [_P, _P, _I, _P]),
     "ap_colorize": (C.c_int, [_P, _VIEW, _P, _P, _I, _P, _P]),
+    "ap_batch_tiles": (C.c_int32, [_P, _I, _I, _I]),
+    "ap_render_fexp32_batch": (C.c_int, [_P, _P, _P, _I, _I, _I, _I, _P, _P]),
 }
 
 _lib = None

@@ -4,6 +4,7 @@
 // ap_color.hip.  Nothing here throws; every failure is an int status with the
 // text kept in the context.
 #include <stdarg.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -261,6 +262,62 @@ int ap_render_fexp64(ap_ctx* c, const ap_view* v, const ap_orbit* orb, const voi
 int ap_render_fexp32(ap_ctx* c, const ap_view* v, const ap_orbit* orb, const void* bla,
                      const ap_outputs* o) {
     return render_fexp(c, 32, v, orb, bla, o);
+}
+
+int32_t ap_batch_tiles(ap_frame* frames, int32_t nframes, int32_t tile_w, int32_t tile_h) {
+    if (!frames || nframes <= 0 || tile_w <= 0 || tile_h <= 0) return 0;
+    long long total = 0;
+    for (int i = 0; i < nframes; ++i) {
+        ap_frame& f = frames[i];
+        f.tiles_x = (f.view.width + tile_w - 1) / tile_w;
+        const long long tiles = (long long)f.tiles_x * ((f.view.height + tile_h - 1) / tile_h);
+        if (f.view.width <= 0 || f.view.height <= 0 || total + tiles > INT32_MAX) return 0;
+        f.tile_begin = (int32_t)total;
+        f.tiles = (int32_t)tiles;
+        total += tiles;
+    }
+    return (int32_t)total;
+}
+
+int ap_render_fexp32_batch(ap_ctx* c, const ap_frame* frames, const ap_frame* frames_dev,
+                           int32_t nframes, int32_t tile_w, int32_t tile_h, int32_t blocks,
+                           int32_t* counter, unsigned long long* stats) {
+    if (!c) return AP_ERR_ARG;
+    if (!frames || !frames_dev || !counter || nframes <= 0)
+        return fail(c, AP_ERR_ARG, "batch needs host and device frame arrays and a counter");
+    if (tile_w <= 0 || tile_h <= 0 || tile_w * tile_h != 256)
+        return fail(c, AP_ERR_ARG, "tile %dx%d: tile_w * tile_h must be 256", tile_w, tile_h);
+    if (blocks < 0) return fail(c, AP_ERR_ARG, "negative block count");
+    const int type = frames[0].view.fractal_type;
+    if (type != AP_MANDELBROT && type != AP_JULIA_MANDELBROT && type != AP_GENERAL_MANDELBROT)
+        return fail(c, AP_ERR_UNSUPPORTED, "batched pass: fractal type %d has no BLA", type);
+    long long next = 0;
+    for (int i = 0; i < nframes; ++i) {
+        const ap_frame& f = frames[i];
+        int r = check_view(c, &f.view, &f.out);
+        if (r) return r;
+        if ((r = check_orbit(c, &f.view, &f.orbit, f.bla))) return r;
+        if (f.view.fractal_type != type) return fail(c, AP_ERR_ARG, "frame %d: mixed fractal types", i);
+        if (f.view.mode != AP_MODE_CENTRE || f.view.factor != 1 || f.view.rebase_pass != 0)
+            return fail(c, AP_ERR_UNSUPPORTED, "frame %d: the batched pass takes centre samples only", i);
+        if (f.out.rgb || f.out.queue || f.out.mask || f.out.stats)
+            return fail(c, AP_ERR_ARG, "frame %d: no rgb, queue, mask or per-frame stats in a batch", i);
+        if (!f.bla || f.view.bla_levels < 2 || f.orbit.len < 4)
+            return fail(c, AP_ERR_ARG, "frame %d: the batched pass needs a BLA table", i);
+        if (f.view.e_scale > 30000 || f.view.e_scale < -30000)
+            return fail(c, AP_ERR_UNSUPPORTED, "frame %d: e_scale %d outside [-30000, 30000]", i, f.view.e_scale);
+        const int tx = (f.view.width + tile_w - 1) / tile_w;
+        const long long tiles = (long long)tx * ((f.view.height + tile_h - 1) / tile_h);
+        if (f.tile_begin != next || f.tiles_x != tx || f.tiles != tiles)
+            return fail(c, AP_ERR_ARG, "frame %d: tile fields disagree with the tile shape (ap_batch_tiles)", i);
+        next += tiles;
+    }
+    if (next > INT32_MAX) return fail(c, AP_ERR_ARG, "batch exceeds 2^31 tiles");
+    int r = use_device(c);
+    if (r) return r;
+    return check(c, ap::launch_fexp32_batch(type, frames_dev, nframes, (int)next, tile_w, tile_h, blocks,
+                                            counter, stats, c->num_cus, c->stream),
+                 "fexp32 batch");
 }
 
 size_t ap_bla_bytes(int32_t ref_len, int32_t levels) {

@@ -1,0 +1,82 @@
+// Batched frames: the FP32 FloatExp pixel passes of many frames in one
+// persistent launch.  Workgroups take 256-pixel tiles from a device counter
+// until the batch's tiles are exhausted; each tile finds its frame by a
+// binary search over the frames' first-tile indices (wave-uniform, so the
+// frame's view is read through the scalar cache).  No reference counterpart
+// (the reference launches one kernel per frame, math_core/engine.py:1776-1806).
+#include "ap_bla.hpp"
+#include "ap_fexp32.hpp"
+#include "ap_launch.hpp"
+#include "ap_pass.hpp"
+
+namespace ap {
+
+__global__ void batch_reset_kernel(int* counter) { *counter = 0; }
+
+template <int T>
+__global__ __launch_bounds__(kBlock) void fexp32_batch_kernel(const ap_frame* __restrict__ frames,
+                                                              int nframes, int total, int tile_w,
+                                                              int tile_h, int* __restrict__ counter,
+                                                              unsigned long long* stats) {
+    __shared__ int s_tile;
+    LaneStats st;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(s_tile);
+        __syncthreads();  // s_tile is rewritten only after every lane has read it
+        if (t >= total) break;  // the counter only grows: every workgroup gets here
+        int lo = 0, hi = nframes - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (frames[mid].tile_begin <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const ap_frame& F = frames[lo];
+        const int local = t - F.tile_begin;
+        const int x = (local % F.tiles_x) * tile_w + (int)threadIdx.x % tile_w;
+        const int y = (local / F.tiles_x) * tile_h + (int)threadIdx.x / tile_w;
+        if (x < F.view.width && y < F.view.height) {
+            Fexp32Data d;
+            d.zr = F.orbit.re;
+            d.zi = F.orbit.im;
+            d.bla = bla_view(F.bla, F.orbit.len, F.view.bla_levels);
+            d.zf = d.bla.zf;
+            process_pixel<EngFexp32<T>, false>(F.view, F.out, d, x, y, true, st);
+        }
+    }
+    flush_stats(stats, st.iters, st.plain, st.bla, st.samples);
+}
+
+template <int T>
+static hipError_t launch_batch_t(const ap_frame* frames_dev, int nframes, int total, int tile_w,
+                                 int tile_h, int blocks, int* counter, unsigned long long* stats,
+                                 int num_cus, hipStream_t s) {
+    if (blocks <= 0) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fexp32_batch_kernel<T>, kBlock, 0) !=
+                hipSuccess ||
+            per_cu <= 0)
+            per_cu = 4;
+        blocks = per_cu * num_cus;
+    }
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, s, counter);
+    hipLaunchKernelGGL(fexp32_batch_kernel<T>, dim3(blocks), dim3(kBlock), 0, s, frames_dev, nframes,
+                       total, tile_w, tile_h, counter, stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_fexp32_batch(int type, const ap_frame* frames_dev, int nframes, int total,
+                               int tile_w, int tile_h, int blocks, int* counter,
+                               unsigned long long* stats, int num_cus, hipStream_t s) {
+    switch (type) {
+    case 0: return launch_batch_t<0>(frames_dev, nframes, total, tile_w, tile_h, blocks, counter, stats, num_cus, s);
+    case 2: return launch_batch_t<2>(frames_dev, nframes, total, tile_w, tile_h, blocks, counter, stats, num_cus, s);
+    case 4: return launch_batch_t<4>(frames_dev, nframes, total, tile_w, tile_h, blocks, counter, stats, num_cus, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace ap

@@ -98,7 +98,7 @@ __host__ __device__ inline BlaLayout bla_layout(int ref_len, int levels) {
     return L;
 }
 
-inline BlaView bla_view(const void* table, int ref_len, int levels) {
+__host__ __device__ inline BlaView bla_view(const void* table, int ref_len, int levels) {
     BlaView b;
     b.ref_len = ref_len;
     if (!table || levels < 2 || ref_len < 4) {

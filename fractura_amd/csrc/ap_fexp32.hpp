@@ -1,0 +1,350 @@
+// The FP32-mantissa FloatExp engine (FEXP32; no reference counterpart: the
+// reference FloatExp mantissa is FP64).  FP64 stays where stability needs it:
+// the near-bailout test and smooth value, Stage 2, and the BLA table build.
+// Used by the per-frame pass (ap_pert.hip) and the batched persistent pass
+// (ap_batch.hip).
+#pragma once
+
+#include "ap_bla.hpp"
+#include "ap_pert_common.hpp"
+
+namespace ap {
+
+// =================================================================== FEXP32
+// State: float mantissa pair (dzx, dzy) and int exponent z_exp, actual delta
+// = dz_m * 2^(z_exp - E).  Z_k is read as float2 from an FP32 copy of the
+// orbit.  A BLA leap leaves max|dz_m| in [1/2, 1); a plain step renormalises
+// by the binary exponent only when the mantissa leaves [2^-32, 2^32] (FP32
+// has the range to wait).  BLA validity compares the ordered int32 key of
+// |dz|^2 with the key of r^2 stored in the entry.
+struct Fexp32Data {
+    const double* zr;
+    const double* zi;
+    const float2* zf;
+    BlaView bla;
+};
+
+// floor(log2 m) of a finite, normal, positive float from its exponent field.
+__device__ inline int exponent_of(float m) { return __builtin_amdgcn_frexp_expf(m) - 1; }
+
+__device__ inline void renorm32(float& dzx, float& dzy, int& z_exp) {
+    float m = fmaxf(fabsf(dzx), fabsf(dzy));
+    if ((m > 4294967296.0f || m < 2.3283064365386963e-10f) && m > 0.0f && isfinite(m)) {
+        int e = exponent_of(m);
+        dzx = ldexpf(dzx, -e);
+        dzy = ldexpf(dzy, -e);
+        z_exp += e;
+    }
+}
+
+// BLA leaps of the FP32 engine.  The highest level a leap from k may take is
+// hi = top_level(k); the entry at level hi is fetched together with the key
+// test, and in deep views it is the one taken, so a leap costs one round trip
+// to the cache.  When it is not valid, leap32_search finds the highest valid
+// level below it (-1 when level 0 is not valid either).
+//
+// dz' = A dz + B dc with dz = m 2^z_exp, A = am 2^ae, B = bm 2^be and dc in
+// units of 2^-E (dc_m = dc 2^E, of order one): the A term has exponent
+// e1 = z_exp + ae + em, 2^em a bound on |m| read off the key's |m|^2, the B
+// term e2 = be.  Both inputs are scaled down to ne = max(e1, e2) before the
+// products, which run as FMAs; the result, at exponent ne, is bounded by a
+// few units (|am| < 2 sqrt 2, scaled |m| < sqrt 2, plus |bm| |dc_m|), so no
+// normalisation step follows.  kNoDc: the B dc term is zero (Julia sets), so
+// its exponent takes no part.
+__device__ inline int leap32_search(const Fexp32Data& D, int k, int hi, int key) {
+    if (!(key < entry(D.bla.c32, (unsigned)k).key)) return -1;
+    return highest_valid_level(hi - 1, [&](int m) { return key < entry(D.bla.c32, D.bla.at(k, m)).key; });
+}
+
+template <bool kNoDc>
+__device__ inline void leap32_terms(const Coef32& c, float q, float dzx, float dzy, int z_exp,
+                                    float dpx, float dpy, float& rx, float& ry, int& ne) {
+    // |m| < 2^em: q = |m|^2 has biased exponent field b, so |m|^2 < 2^(b - 126)
+    // and em = (b >> 1) - 62 bounds it (q = 0 gives a small em).
+    const int em = (int)(__float_as_uint(q) >> 24) - 62;
+    const int za = z_exp + c.ae;
+    if (kNoDc) {
+        ne = za + em;
+        const float mx = ldexpf(dzx, -em), my = ldexpf(dzy, -em);
+        rx = fmaf(c.ar, mx, -c.ai * my);
+        ry = fmaf(c.ar, my, c.ai * mx);
+    } else {
+        ne = max(za + em, c.be);
+        const float mx = ldexpf(dzx, za - ne), my = ldexpf(dzy, za - ne);
+        const float px = ldexpf(dpx, c.be - ne), py = ldexpf(dpy, c.be - ne);
+        rx = fmaf(c.ar, mx, fmaf(-c.ai, my, fmaf(c.br, px, -c.bi * py)));
+        ry = fmaf(c.ar, my, fmaf(c.ai, mx, fmaf(c.br, py, c.bi * px)));
+    }
+}
+
+// One leap from k if level 0 is valid: returns the level taken or -1.
+template <bool kNoDc>
+__device__ inline int bla_leap32(const Fexp32Data& D, float& dzx, float& dzy, int& z_exp,
+                                 float dpx, float dpy, int k, int hi, int E) {
+    Coef32 c = entry(D.bla.c32, D.bla.at(k, hi));
+    const float q = fmaf(dzx, dzx, dzy * dzy);
+    const int key = dz_key(q, z_exp - E);
+    int d = hi;
+    if (!(key < c.key)) {
+        d = leap32_search(D, k, hi, key);
+        if (d < 0) return -1;
+        c = entry(D.bla.c32, D.bla.at(k, d));
+    }
+    float rx, ry;
+    int ne;
+    leap32_terms<kNoDc>(c, q, dzx, dzy, z_exp, dpx, dpy, rx, ry, ne);
+    dzx = rx;
+    dzy = ry;
+    z_exp = ne;
+    return d;
+}
+
+// Consecutive leaps while z_exp < z_deep, in a loop with a single exit and
+// a single (rare) branch, so the exec-mask bookkeeping per leap stays small:
+// a lane whose level 0 is not valid at its k applies the identity entry
+// (A = 1, B = 0, which rescales its mantissa exactly) with next_hi = -2, and
+// leaves the loop; the -2 then sets no_leap.
+template <bool kNoDc>
+__device__ inline void deep_leaps32(const Fexp32Data& D, float& dzx, float& dzy, int& z_exp,
+                                    float dpx, float dpy, int& k, int& nbla, bool& no_leap,
+                                    int E, int z_deep, int ref_len) {
+    int hi = top_level(k, ref_len, D.bla.levels);
+    while (z_exp < z_deep && hi >= 0) {
+        Coef32 c = entry(D.bla.c32, D.bla.at(k, hi));
+        const float q = fmaf(dzx, dzx, dzy * dzy);
+        const int key = dz_key(q, z_exp - E);
+        int d = hi;
+        if (!(key < c.key)) {
+            d = leap32_search(D, k, hi, key);
+            if (d >= 0) {
+                c = entry(D.bla.c32, D.bla.at(k, d));
+            } else {
+                c.ar = 1.0f;
+                c.ai = c.br = c.bi = 0.0f;
+                c.ae = 0;
+                c.be = kZeroExp;
+                c.next_hi = -2;
+            }
+        }
+        float rx, ry;
+        int ne;
+        leap32_terms<kNoDc>(c, q, dzx, dzy, z_exp, dpx, dpy, rx, ry, ne);
+        dzx = rx;
+        dzy = ry;
+        z_exp = ne;
+        k += d >= 0 ? 1 << d : 0;
+        nbla += d >= 0 ? 1 : 0;
+        hi = c.next_hi;
+    }
+    no_leap = hi == -2;
+}
+
+// Hand-off point of the FP32 loop: once |dz| >= 2^kTail32 the delta continues
+// in FP64 with plain steps.  Near escape a chaotic excursion amplifies the
+// error already in dz by the product of |2z| over the excursion; a 24-bit
+// mantissa, and the 1e-6 error of BLA leaps taken that late, do not survive
+// it.  Below 2^-32 the delta is a linear correction whose relative error does
+// not grow, so the FP32 loop and its BLA leaps cover all the deep iterations.
+// (Measured on seeded deep views: p99 smooth error against exact FP64
+// perturbation 1.4e-5..1.8e-5, versus 3e-4..4e-4 for the FP64-mantissa loop
+// with BLA to the end.)
+constexpr int kTail32 = -32;
+
+// FP64 perturbation continuation of the FP32 loop from iteration k, with the
+// delta now in actual units (dc * 2^-E underflows to 0 beyond FP64 range,
+// where it is negligible next to |dz| >= 2^kTail32).
+template <int T>
+__device__ Sample fexp32_tail(const ap_view& v, const Fexp32Data& D, const PertConsts& pc,
+                              double dzx, double dzy, double dcx_d, double dcy_d, int k,
+                              int glitch, int plain, int nbla) {
+    const bool julia = (T == 2 || T == 3);
+    const int ref_len = v.ref_len;
+    const bool strict = v.strict_glitch > 0;
+    const double sc_e = v.e_scale <= 1074 ? ldexp(1.0, -v.e_scale) : 0.0;
+    const double dpx = julia ? 0.0 : dcx_d * sc_e, dpy = julia ? 0.0 : dcy_d * sc_e;
+    const int n = v.gen_n;
+    while (k < ref_len) {
+        const double rx = D.zr[k], ry = D.zi[k];
+        const double zx = rx + dzx, zy = ry + dzy;
+        const double mag2 = zx * zx + zy * zy;
+        if (mag2 > kBailout2) {
+            Sample r = escaped(k, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
+            r.glitch = glitch;
+            r.plain = plain;
+            r.bla = nbla;
+            return r;
+        }
+        if (strict && k > 15) {
+            double r_mag2 = rx * rx + ry * ry;
+            if (mag2 < 1e-6 * r_mag2 && r_mag2 > 1e-8) glitch = 1;
+        }
+        if (T == 0 || T == 2) {
+            double ndx = (2.0 * rx + dzx) * dzx - (2.0 * ry + dzy) * dzy + dpx;
+            dzy = 2.0 * ((rx + dzx) * dzy + ry * dzx) + dpy;
+            dzx = ndx;
+        } else if (T == 1 || T == 3) {
+            double ndx = (2.0 * rx + dzx) * dzx - (2.0 * ry + dzy) * dzy + dpx;
+            double sxs = rx >= 0.0 ? 1.0 : -1.0;
+            double sys = ry >= 0.0 ? 1.0 : -1.0;
+            double ddx = (rx + dzx) * sxs >= 0.0 ? sxs * dzx : -sxs * (2.0 * rx + dzx);
+            double ddy = (ry + dzy) * sys >= 0.0 ? sys * dzy : -sys * (2.0 * ry + dzy);
+            dzy = 2.0 * (fabs(rx) * ddy + fabs(ry) * ddx + ddx * ddy) - dpy;
+            dzx = ndx;
+        } else {
+            double pr = dzx, pi = dzy, qr = rx, qi = ry;
+            for (int e = 0; e < n - 1; ++e) {
+                double t1r = pr * rx - pi * ry, t1i = pr * ry + pi * rx;
+                double sr = pr + qr, sim = pi + qi;
+                double t2r = sr * dzx - sim * dzy, t2i = sr * dzy + sim * dzx;
+                pr = t1r + t2r;
+                pi = t1i + t2i;
+                double nq = qr * rx - qi * ry;
+                qi = qr * ry + qi * rx;
+                qr = nq;
+            }
+            double kdr = v.gen_kr * dzx - v.gen_ki * dzy, kdi = v.gen_kr * dzy + v.gen_ki * dzx;
+            dzx = pr + kdr + dpx;
+            dzy = pi + kdi + dpy;
+        }
+        ++k;
+        ++plain;
+    }
+    const double zx = D.zr[ref_len] + dzx, zy = D.zi[ref_len] + dzy;
+    const double scfb = v.e_scale <= 1020 ? pow2_f64(-v.e_scale) : 0.0;
+    const double cx = julia ? v.julia_cx : v.ref_cx + dcx_d * scfb;
+    const double cy = julia ? v.julia_cy : v.ref_cy + dcy_d * scfb;
+    const double mag2 = zx * zx + zy * zy;
+    if (mag2 > kBailout2) {
+        Sample r = escaped(ref_len, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
+        r.glitch = glitch;
+        r.plain = plain;
+        r.bla = nbla;
+        return r;
+    }
+    if (ref_len < v.max_iter) glitch = 1;
+    return stage2<T>(v, pc, zx, zy, cx, cy, k, glitch, plain, nbla);
+}
+
+template <int T>
+__device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si, double sj) {
+    const bool julia = (T == 2 || T == 3);
+    const double dcy_d = (v.ref_py - sj) * v.dy;
+    const double dcx_d = (si - v.ref_px) * v.dx;
+    const float dcx = (float)dcx_d, dcy = (float)dcy_d;
+    float dzx = julia ? dcx : 0.0f, dzy = julia ? dcy : 0.0f;
+    const float dpx = julia ? 0.0f : dcx, dpy = julia ? 0.0f : dcy;
+    int z_exp = 0;
+    renorm32(dzx, dzy, z_exp);
+    const int E = v.e_scale;
+    const int ref_len = v.ref_len;
+    const bool strict = v.strict_glitch > 0;
+    const bool has_bla = D.bla.levels > 1 && (T == 0 || T == 2 || T == 4);
+    const PertConsts pc = pert_consts(v);
+    int k = 0, glitch = 0, plain = 0, nbla = 0;
+    const int n = v.gen_n;
+    const float kr = (float)v.gen_kr, ki = (float)v.gen_ki;
+    const int z_deep = E + kTail32 - 40;
+    if (mandel_start<T>(v, D.zr, D.zi)) {
+        dzx = dpx;
+        dzy = dpy;
+        renorm32(dzx, dzy, z_exp);
+        k = plain = 1;
+    }
+
+    while (k < ref_len) {
+        // Deep regime (z_exp < z_deep, i.e. eff < kTail32 - 40): max|dz_m| < 2^32,
+        // so |dz| < 2^(eff + 32) < 2^(kTail32 - 8): no hand-off, no glitch test
+        // (it needs eff >= -40) and no escape can happen; only BLA leaps run,
+        // in a loop of their own.  A leap of finite mantissas (inputs scaled
+        // to at most 1, |am|, |bm| < 2) is finite, so the loop has no NaN test.
+        bool no_leap = false;  // level 0 already failed at this k and state
+        if ((T == 0 || T == 2 || T == 4) && has_bla) {
+            deep_leaps32<julia>(D, dzx, dzy, z_exp, dpx, dpy, k, nbla, no_leap, E, z_deep, ref_len);
+            if (k >= ref_len) break;
+        }
+        const int eff = z_exp - E;
+        // |dz| = max|dz_m| 2^eff >= 2^kTail32: continue in FP64.
+        if (eff >= kTail32 - 40) {
+            float m = fmaxf(fabsf(dzx), fabsf(dzy));
+            if (m > 0.0f && eff + ilogbf(m) >= kTail32)
+                return fexp32_tail<T>(v, D, pc, ldexp((double)dzx, eff), ldexp((double)dzy, eff),
+                                      dcx_d, dcy_d, k, glitch, plain, nbla);
+        }
+        const float2 Z = D.zf[k];
+        if (strict && k > 15 && eff >= -40) {
+            // |Z + dz|^2 < 1e-6 |Z|^2 in actual units, for |Z| >= 1e-4.
+            float rmax = fmaxf(fabsf(Z.x), fabsf(Z.y));
+            if (rmax >= 1e-4f && (dzx != 0.0f || dzy != 0.0f)) {
+                float zx = Z.x + ldexpf(dzx, eff), zy = Z.y + ldexpf(dzy, eff);
+                float m2 = zx * zx + zy * zy;
+                float rm2 = Z.x * Z.x + Z.y * Z.y;
+                if (m2 < 1e-6f * rm2) glitch = 1;
+            }
+        }
+        if (T == 0 || T == 2 || T == 4) {
+            if (has_bla && !no_leap && k < ref_len - 1) {
+                const int d = bla_leap32<julia>(D, dzx, dzy, z_exp, dpx, dpy, k,
+                                                top_level(k, ref_len, D.bla.levels), E);
+                if (d >= 0) {
+                    ++nbla;
+                    k += 1 << d;
+                    continue;
+                }
+            }
+        }
+        const float s_nl = ldexpf(1.0f, max(z_exp - E, -200));
+        const float cxs = ldexpf(dpx, max(-z_exp, -200)), cys = ldexpf(dpy, max(-z_exp, -200));
+        if (T == 0 || T == 2) {
+            // 2 Z dz + dz^2 2^(z_exp-E) + dc 2^-z_exp
+            float tx = dzx * s_nl, ty = dzy * s_nl;
+            float lx = 2.0f * (Z.x * dzx - Z.y * dzy) + (tx * dzx - ty * dzy) + cxs;
+            float ly = 2.0f * (Z.x * dzy + Z.y * dzx) + 2.0f * tx * dzy + cys;
+            dzx = lx;
+            dzy = ly;
+        } else if (T == 1 || T == 3) {
+            float sxs = Z.x >= 0.0f ? 1.0f : -1.0f;
+            float sys = Z.y >= 0.0f ? 1.0f : -1.0f;
+            float ax = ldexpf(dzx, max(z_exp - E, -200)), ay = ldexpf(dzy, max(z_exp - E, -200));
+            float back = ldexpf(1.0f, min(E - z_exp, 200));
+            float ddx = (Z.x + ax) * sxs >= 0.0f ? sxs * dzx : -sxs * (2.0f * Z.x * back + dzx);
+            float ddy = (Z.y + ay) * sys >= 0.0f ? sys * dzy : -sys * (2.0f * Z.y * back + dzy);
+            float lx = (2.0f * Z.x + dzx * s_nl) * dzx - (2.0f * Z.y + dzy * s_nl) * dzy + cxs;
+            float ly = 2.0f * (fabsf(Z.x) * ddy + fabsf(Z.y) * ddx + ddx * ddy * s_nl) - cys;
+            dzx = lx;
+            dzy = ly;
+        } else {
+            float pr = dzx, pi = dzy, qr = Z.x, qi = Z.y;
+            for (int e = 0; e < n - 1; ++e) {
+                float t1r = pr * Z.x - pi * Z.y, t1i = pr * Z.y + pi * Z.x;
+                float t2r = qr * dzx - qi * dzy, t2i = qr * dzy + qi * dzx;
+                float t3r = pr * dzx - pi * dzy, t3i = pr * dzy + pi * dzx;
+                pr = t1r + t2r + t3r * s_nl;
+                pi = t1i + t2i + t3i * s_nl;
+                float nq = qr * Z.x - qi * Z.y;
+                qi = qr * Z.y + qi * Z.x;
+                qr = nq;
+            }
+            float ndx = pr + (kr * dzx - ki * dzy) + cxs;
+            dzy = pi + (kr * dzy + ki * dzx) + cys;
+            dzx = ndx;
+        }
+        renorm32(dzx, dzy, z_exp);
+        ++k;
+        ++plain;
+    }
+    // The reference escaped with the delta still tiny: Stage 2 from
+    // Z_ref_len + dz in FP64.
+    return fexp32_tail<T>(v, D, pc, ldexp((double)dzx, z_exp - E), ldexp((double)dzy, z_exp - E),
+                          dcx_d, dcy_d, k, glitch, plain, nbla);
+}
+
+template <int T>
+struct EngFexp32 {
+    using Data = Fexp32Data;
+    static constexpr bool kIterInSubsamples = true;
+    __device__ static Sample sample(const ap_view& v, const Data& d, double si, double sj) {
+        return fexp32_sample<T>(v, d, si, sj);
+    }
+};
+
+}  // namespace ap

@@ -17,6 +17,11 @@ hipError_t launch_pert64(const ap_view& v, const ap_orbit& orb, const void* bla,
 hipError_t launch_fexp(int mant, const ap_view& v, const ap_orbit& orb, const void* bla,
                        const ap_outputs& o, void* scratch_f32_orbit, hipStream_t s);
 
+// Batched FP32 FloatExp frames in one persistent launch (types 0, 2, 4).
+hipError_t launch_fexp32_batch(int type, const ap_frame* frames_dev, int nframes, int total,
+                               int tile_w, int tile_h, int blocks, int* counter,
+                               unsigned long long* stats, int num_cus, hipStream_t s);
+
 // BLA table.  Limits: ref_len < 2^24 (24-bit index multiplies) and at most
 // 2^27 entries (32-bit byte offsets into the 32-byte region).
 constexpr int kMaxBlaRefLen = (1 << 24) - 1;

@@ -1,0 +1,155 @@
+// Pieces shared by the perturbation engines: the reference's pow2 and
+// Pauldelbrot helpers, FP64 renormalisation, the NaN exit, the k = 0 start
+// of Mandelbrot-type orbits, and Stage 2 (direct FP64 once the reference has
+// escaped).  Restated from math_core/cpu_kernels.py:161-201, 373-602, 605-1021.
+#pragma once
+
+#include "ap_device.hpp"
+
+namespace ap {
+
+constexpr double kInv2p16 = 1.52587890625e-05;  // 2^-16
+
+// pow2_f64 of the reference (math_core/cpu_kernels.py:161-167): 2^e with
+// e >= 1023 -> DBL_MAX and e <= -1022 -> 0.
+__device__ inline double pow2_f64(int e) {
+    if (e >= 1023) return 1.7976931348623157e+308;
+    if (e <= -1022) return 0.0;
+    return ldexp(1.0, e);
+}
+
+// Python max(a, b) (returns a unless b > a; matters only for NaN).
+__device__ inline double pymax(double a, double b) { return b > a ? b : a; }
+
+// int(log(x) / ln 2) for x > 0, computed from the binary exponent: truncation
+// toward zero of log2 x.  Equal to the reference's log-based value except
+// when x lies within rounding distance of a power of two.
+__device__ inline int trunc_log2(double x) {
+    int e;
+    double m = frexp(x, &e);  // x = m 2^e, m in [0.5, 1)
+    if (x >= 1.0) return e - 1;
+    return m == 0.5 ? e - 1 : e;
+}
+
+// Pauldelbrot test in rescaled exponent space
+// (check_glitch_cancellation_floatexp, math_core/cpu_kernels.py:170-201).
+__device__ inline int glitch_floatexp(double rx, double ry, double dzx, double dzy, int eff) {
+    double r_max = pymax(fabs(rx), fabs(ry));
+    if (r_max < 1e-4) return 0;
+    double dz_max = pymax(fabs(dzx), fabs(dzy));
+    if (dz_max <= 0.0) return 0;
+    int r_exp = trunc_log2(r_max);
+    int dz_exp = trunc_log2(dz_max);
+    int diff = r_exp - (eff + dz_exp);
+    if (diff > 10 || diff < -10) return 0;
+    double sr = pow2_f64(-r_exp), sd = pow2_f64(eff - r_exp);
+    double rxs = rx * sr, rys = ry * sr;
+    double zx = rxs + dzx * sd, zy = rys + dzy * sd;
+    double m2 = zx * zx + zy * zy;
+    double rm2 = rxs * rxs + rys * rys;
+    return m2 < 1e-6 * rm2 ? 1 : 0;
+}
+
+// Keep max(|dzx|, |dzy|) inside [2^-16, 2^16] in steps of 2^16.
+__device__ inline void renorm64(double& dzx, double& dzy, int& z_exp) {
+    double mag = pymax(fabs(dzx), fabs(dzy));
+    while (mag > 65536.0 && isfinite(mag)) {
+        dzx *= kInv2p16;
+        dzy *= kInv2p16;
+        z_exp += 16;
+        mag = pymax(fabs(dzx), fabs(dzy));
+    }
+    while (mag < kInv2p16 && mag > 0.0 && isfinite(mag)) {
+        dzx *= 65536.0;
+        dzy *= 65536.0;
+        z_exp -= 16;
+        mag = pymax(fabs(dzx), fabs(dzy));
+    }
+}
+
+__device__ inline Sample nan_exit(int k) {
+    Sample r;
+    r.count = k;
+    r.state = kNanExit;
+    r.mag2 = 0.0;
+    r.glitch = 1;
+    r.plain = 0;
+    r.bla = 0;
+    return r;
+}
+
+// Step 0 of a Mandelbrot-type orbit (types 0 and 4, Z_0 = 0, dz_0 = 0) gives
+// dz_1 = dc exactly: every engine's step formula reduces to it, and neither
+// the escape test (|Z_0|^2 = 0) nor the glitch test (k > 15) fires at k = 0.
+// The engines start such pixels at k = 1; the skipped step counts as plain.
+template <int T>
+__device__ inline bool mandel_start(const ap_view& v, const double* zr, const double* zi) {
+    return (T == 0 || T == 4) && v.ref_len >= 1 && v.max_iter >= 1 && zr[0] == 0.0 && zi[0] == 0.0;
+}
+
+struct PertConsts {
+    double inv_ref, inv_exact;
+};
+
+__device__ inline PertConsts pert_consts(const ap_view& v) {
+    PertConsts c;
+    c.inv_ref = c.inv_exact = v.inv_ln2;
+    if (v.fractal_type == AP_GENERAL_MANDELBROT && v.gen_n > 1) {
+        c.inv_exact = 1.0 / log((double)v.gen_n);
+        c.inv_ref = (double)(float)c.inv_exact;
+    }
+    return c;
+}
+
+// Stage 2 of every perturbation engine: the reference escaped (or ran out),
+// so the pixel continues as a direct FP64 iteration of Z_ref_len + dz.
+template <int T>
+__device__ inline Sample stage2(const ap_view& v, const PertConsts& pc, double zx, double zy,
+                                double cx, double cy, int k, int glitch, int plain, int bla) {
+    const int max_iter = v.max_iter;
+    const int n = v.gen_n;
+    while (k < max_iter) {
+        double zx2 = zx * zx, zy2 = zy * zy;
+        double mag2 = zx2 + zy2;
+        if (isnan(mag2) || isinf(mag2)) {
+            Sample r = nan_exit(k);
+            r.plain = plain;
+            r.bla = bla;
+            return r;
+        }
+        if (mag2 > kBailout2) {
+            Sample r = escaped(k, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
+            r.glitch = glitch;
+            r.plain = plain;
+            r.bla = bla;
+            return r;
+        }
+        if (T == 0 || T == 2) {
+            zy = 2.0 * zx * zy + cy;
+            zx = zx2 - zy2 + cx;
+        } else if (T == 1 || T == 3) {
+            zy = 2.0 * fabs(zx) * fabs(zy) - cy;
+            zx = zx2 - zy2 + cx;
+        } else {
+            double pr = zx, pi = zy;
+            for (int e = 0; e < n - 1; ++e) {
+                double t = pr * zx - pi * zy;
+                pi = pr * zy + pi * zx;
+                pr = t;
+            }
+            double kzr = v.gen_kr * zx - v.gen_ki * zy;
+            double kzi = v.gen_kr * zy + v.gen_ki * zx;
+            zx = pr + kzr + cx;
+            zy = pi + kzi + cy;
+        }
+        ++k;
+        ++plain;
+    }
+    Sample r = interior(max_iter, max_iter);
+    r.glitch = glitch;
+    r.plain = plain;
+    r.bla = bla;
+    return r;
+}
+
+}  // namespace ap

@@ -131,6 +131,21 @@ typedef struct ap_orbit {
     int32_t reserved0;
 } ap_orbit;
 
+/* One frame of a batched pass (ap_render_fexp32_batch): the frame's view,
+ * its orbit and BLA table (built with AP_BLA_FORM_FP32), its output maps, and
+ * its place in the batch's tile sequence (ap_batch_tiles fills the last
+ * four fields). */
+typedef struct ap_frame {
+    ap_view view;         /* mode 0 (centre sample), factor 1               */
+    ap_orbit orbit;
+    const void *bla;      /* BLA table, required                            */
+    ap_outputs out;       /* counts / smooth / iter_map / glitch; no rgb    */
+    int32_t tile_begin;   /* index of the frame's first tile                */
+    int32_t tiles_x;      /* tiles per row = ceil(width / tile_w)           */
+    int32_t tiles;        /* tiles of the frame                             */
+    int32_t reserved0;
+} ap_frame;
+
 /* ---- context, stream and memory ------------------------------------------
  * Replaces init_primary_cuda_device (math_core/cuda_kernels.py:11-30), the
  * Numba stream/device-array plumbing and the scratch pool
@@ -174,6 +189,25 @@ int ap_render_fexp64(ap_ctx *ctx, const ap_view *view, const ap_orbit *orbit,
                      const void *bla, const ap_outputs *out);
 int ap_render_fexp32(ap_ctx *ctx, const ap_view *view, const ap_orbit *orbit,
                      const void *bla, const ap_outputs *out);
+
+/* ---- batched frames ------------------------------------------------------
+ * No reference counterpart: the reference renders one frame per launch
+ * (math_core/engine.py:1742-1806 for FloatExp) and a video one frame at a
+ * time (workers/video_workers.py).  Here the pixel passes of many frames of
+ * uneven size and depth run as one persistent launch: `blocks` workgroups
+ * (0 = as many as fit on the device at once) take tiles of tile_w x tile_h
+ * pixels (tile_w * tile_h = 256) from a device counter until the batch's
+ * tiles are exhausted, so a deep frame's slow tiles and a shallow frame's
+ * fast ones share the device without a tail per frame.  The frames array is
+ * read by the device (frames_dev) and checked on the host (frames, the same
+ * contents); every frame has the same fractal type (0, 2 or 4).
+ * ap_batch_tiles fills tile_begin / tiles_x / tiles in array order and
+ * returns the total (callers put the slowest frames first).  *counter is
+ * device scratch of one int32 (the library resets it on the stream). */
+int32_t ap_batch_tiles(ap_frame *frames, int32_t nframes, int32_t tile_w, int32_t tile_h);
+int ap_render_fexp32_batch(ap_ctx *ctx, const ap_frame *frames, const ap_frame *frames_dev,
+                           int32_t nframes, int32_t tile_w, int32_t tile_h, int32_t blocks,
+                           int32_t *counter, unsigned long long *stats);
 
 /* ---- chained BLA table -----------------------------------------------------
  * Replaces _jit_build_bla_table (math_core/high_precision.py:1100-1238) and

@@ -1,0 +1,146 @@
+"""Write tests/golden/mpmath_subsamples.json: escape counts and smooth values of
+seeded pixel subsamples, iterated directly in mpmath at high precision.
+
+This is the accuracy oracle of the deep-zoom engines: it shares no code with
+the reference orbit, the perturbation loops or the CPU restatement.  Each
+pixel's c is the exact grid point of its view,
+
+    c = (cx - w/2 + x w/(W-1)) + i (cy + h/2 - y h/(H-1)),   h = w H / W
+
+(math_core/engine.py:493-511 with factor 1), evaluated in Decimal and mpmath
+at the working precision.  The iteration and escape test follow
+cpu_compute_fp64_sample (math_core/cpu_kernels.py:227-266): z_0 = 0,
+escape at the first k with |z_k|^2 > 65536, count = k, interior = max_iter;
+Burning Ship uses the reference's sign convention zy' = 2|zx||zy| - cy, and
+type 4 is z^n + k z + c with k the decimal value of its float repr.  smooth = k + 1 - log_n(ln|z_k| / ln 256) (the
+float64 smooth potential of the library).
+
+Run from the repository root (about five minutes on 8 cores); names given on
+the command line regenerate only those views and keep the others:
+
+    python tests/golden/make_mpmath_fixtures.py [view names]
+"""
+import json
+import math
+import os
+import sys
+import time
+from decimal import Decimal, getcontext
+from multiprocessing import Pool
+
+import mpmath
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "mpmath_subsamples.json")
+BAILOUT = 65536
+
+
+def views():
+    with open(os.path.join(HERE, "configs.json")) as f:
+        cfg = json.load(f)
+    c2, c3, c4 = cfg["config2"], cfg["config3"], cfg["config4"]
+    getcontext().prec = 400
+    return [
+        # name, cx, cy, width, W, H, max_iter, type, gen (n, kr, ki), digits, samples
+        dict(name="deep_misiurewicz_i_1e-320", cx="0", cy="1", width="1e-320", W=640, H=360,
+             max_iter=5000, type=0, digits=360, samples=1024),
+        dict(name="deep_misiurewicz_i_1e-1000", cx="0", cy="1", width="1e-1000", W=640, H=360,
+             max_iter=5000, type=0, digits=1050, samples=1024),
+        dict(name="config3", cx=c3["center_x"], cy=c3["center_y"], width=c3["width"],
+             W=c3["size"][0], H=c3["size"][1], max_iter=c3["max_iter"], type=0, digits=1050,
+             samples=24),
+        dict(name="config2", cx=c2["center_x"], cy=c2["center_y"], width=c2["width"],
+             W=c2["size"][0], H=c2["size"][1], max_iter=c2["max_iter"], type=0, digits=200,
+             samples=4096),
+        dict(name="config4", cx=c4["center_x"], cy=c4["center_y"], width=c4["width"],
+             W=c4["size"][0], H=c4["size"][1], max_iter=c4["max_iter"], type=1, digits=120,
+             samples=1024),
+    ] + [config5_frame(cfg["config5"], f) for f in (12, 31, 47, 63)]
+
+
+def config5_frame(c5, f):
+    """Frame f of the config-5 fly-through (z^3 + k z + c)."""
+    fr = c5["frames"][f]
+    depth = max(0, -Decimal(fr["width"]).adjusted())
+    return dict(name=f"config5_frame{f}", cx=fr["center_x"], cy=fr["center_y"], width=fr["width"],
+                W=fr["size"][0], H=fr["size"][1], max_iter=fr["max_iter"], type=4,
+                gen=(c5["gen_n"], c5["gen_kr"], c5["gen_ki"]), digits=depth + 40, samples=512)
+
+
+def pixel_c(view, x, y):
+    getcontext().prec = view["digits"] + 30
+    cx, cy, w = Decimal(view["cx"]), Decimal(view["cy"]), Decimal(view["width"])
+    W, H = view["W"], view["H"]
+    h = w * Decimal(H) / Decimal(W)
+    re = cx - w / 2 + Decimal(x) * w / Decimal(W - 1)
+    im = cy + h / 2 - Decimal(y) * h / Decimal(H - 1)
+    return str(re), str(im)
+
+
+def escape(args):
+    view, x, y = args
+    mpmath.mp.prec = int(view["digits"] * 3.33) + 64
+    mpf = mpmath.mpf
+    sre, sim = pixel_c(view, x, y)
+    cx, cy = mpf(sre), mpf(sim)
+    t = view["type"]
+    n, kr, ki = view.get("gen", (2, 0.0, 0.0))
+    # k as the decimal value of its float repr, as the reference's orbit takes it
+    # (mpfr(str(gen_kr)), math_core/high_precision.py:728-729).
+    kr, ki = mpf(repr(float(kr))), mpf(repr(float(ki)))
+    zx, zy = mpf(0), mpf(0)
+    for k in range(view["max_iter"]):
+        x2, y2 = zx * zx, zy * zy
+        m2 = x2 + y2
+        if m2 > BAILOUT:
+            lnz = mpmath.log(m2) / 2
+            base = math.log(n) if t == 4 else math.log(2)
+            smooth = (k + 1) - float(mpmath.log(lnz / mpmath.log(256))) / base
+            return int(k), smooth
+        if t == 0:
+            zx, zy = x2 - y2 + cx, 2 * zx * zy + cy
+        elif t == 1:
+            zx, zy = x2 - y2 + cx, 2 * abs(zx) * abs(zy) - cy
+        else:
+            pr, pi = zx, zy
+            for _ in range(n - 1):
+                pr, pi = pr * zx - pi * zy, pr * zy + pi * zx
+            zx, zy = pr + kr * zx - ki * zy + cx, pi + kr * zy + ki * zx + cy
+    return int(view["max_iter"]), -1.0
+
+
+def main():
+    seed = 20261203
+    only = set(sys.argv[1:])
+    out = {"seed": seed, "bailout": BAILOUT, "views": []}
+    old = {}
+    if only and os.path.exists(OUT):
+        with open(OUT) as f:
+            old = {v["name"]: v for v in json.load(f)["views"]}
+    with Pool(min(8, os.cpu_count() or 1)) as pool:
+        for i, view in enumerate(views()):
+            if only and view["name"] not in only and view["name"] in old:
+                out["views"].append(old[view["name"]])
+                continue
+            rng = np.random.Generator(np.random.PCG64(seed + 1000 + i))
+            n = view["samples"]
+            xs = rng.integers(0, view["W"], n)
+            ys = rng.integers(0, view["H"], n)
+            t0 = time.time()
+            res = pool.map(escape, [(view, int(x), int(y)) for x, y in zip(xs, ys)], chunksize=4)
+            counts = [r[0] for r in res]
+            smooth = [r[1] for r in res]
+            print(f"{view['name']}: {n} px, {time.time() - t0:.0f}s, "
+                  f"interior {np.mean(np.array(counts) >= view['max_iter']):.3f}", flush=True)
+            entry = dict(view)
+            entry.update(x=[int(v) for v in xs], y=[int(v) for v in ys], counts=counts,
+                         smooth=[float(s) for s in smooth])
+            out["views"].append(entry)
+    with open(OUT, "w") as f:
+        json.dump(out, f, separators=(",", ":"))
+    print("wrote", OUT, os.path.getsize(OUT), "bytes")
+
+
+if __name__ == "__main__":
+    sys.exit(main())

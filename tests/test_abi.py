@@ -15,6 +15,10 @@ def declared_symbols():
     return sorted(set(re.findall(r"\b(ap_[a-z0-9_]+)\s*\(", text)))
 
 
+STRUCTS = (("ap_view", _abi.ApView), ("ap_outputs", _abi.ApOutputs), ("ap_orbit", _abi.ApOrbit),
+           ("ap_frame", _abi.ApFrame))
+
+
 def test_header_declares_the_path_entry_points():
     syms = declared_symbols()
     for s in ("ap_render_fp64", "ap_render_dd", "ap_render_pert64", "ap_render_fexp64",
@@ -65,7 +69,7 @@ def test_ctypes_layout_matches_the_c_compiler(tmp_path):
     if shutil.which("gcc") is None:
         pytest.skip("no gcc")
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "apeiron_hip.h"', "int main(void){"]
-    for st, cls in (("ap_view", _abi.ApView), ("ap_outputs", _abi.ApOutputs), ("ap_orbit", _abi.ApOrbit)):
+    for st, cls in STRUCTS:
         lines.append(f'printf("{st} size %zu\\n", sizeof({st}));')
         for name, _ in cls._fields_:
             lines.append(f'printf("{st} {name} %zu\\n", offsetof({st}, {name}));')
@@ -76,7 +80,21 @@ def test_ctypes_layout_matches_the_c_compiler(tmp_path):
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
     got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
-    for st, cls in (("ap_view", _abi.ApView), ("ap_outputs", _abi.ApOutputs), ("ap_orbit", _abi.ApOrbit)):
+    for st, cls in STRUCTS:
         assert got[(st, "size")] == C.sizeof(cls)
         for name, _ in cls._fields_:
             assert got[(st, name)] == getattr(cls, name).offset, (st, name)
+
+
+def test_batch_tiles_is_pure_host_arithmetic():
+    lib = _abi.load()
+    frames = (_abi.ApFrame * 3)()
+    for f, (w, h) in zip(frames, [(256, 256), (640, 360), (17, 5)]):
+        f.view.width, f.view.height = w, h
+    total = lib.ap_batch_tiles(frames, 3, 16, 16)
+    assert [f.tiles for f in frames] == [256, 40 * 23, 2]
+    assert [f.tile_begin for f in frames] == [0, 256, 256 + 920]
+    assert [f.tiles_x for f in frames] == [16, 40, 2]
+    assert total == 256 + 920 + 2
+    assert lib.ap_batch_tiles(frames, 3, 32, 8) == 8 * 32 + 20 * 45 + 1 * 1
+    assert lib.ap_render_fexp32_batch(None, frames, frames, 3, 16, 16, 0, None, None) == _abi.AP_ERR_ARG
