@@ -1,6 +1,6 @@
 """Throughput benchmark of the escape-time hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload deep|fp64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload deep|fp64|flythrough]
 
 Workloads (inputs seeded, tests/golden/configs.json):
   deep  3840x2160 Mandelbrot perturbation at width 1e-1000, max_iter 200000,
@@ -10,6 +10,10 @@ Workloads (inputs seeded, tests/golden/configs.json):
         reference's FloatExp design) is timed on the same orbit and table.
   fp64  Plain FP64 Mandelbrot at (-0.65, 0), width 4, max_iter 1000,
         3840x2160 (config 1's view at 4K).
+  flythrough  Config 5: 64 frames of z^3 + k z + c zooming from width 4 to
+        1e-200 (tiles 256x256 to 1920x1080, max_iter 1e3 to 1e5), all on the
+        FP32 FloatExp + BLA path in one persistent launch, the batch captured
+        as one HIP graph.
 A step is one frame of device work with the inputs (the reference orbit)
 resident in HBM: the BLA table build from the orbit (deep), the pixel pass
 and the palette colouring, captured as one HIP graph and replayed.  The host
@@ -409,6 +413,106 @@ def run_fp64(args, torch):
     }
 
 
+def prepare_flythrough(args):
+    """Host side of the config-5 fly-through, before the GPU is touched: the
+    64 frames' reference orbits, computed in a forked process pool."""
+    from fractura_amd import batch
+
+    cfg5 = load_configs()["config5"]
+    specs = batch.flythrough_specs(cfg5)
+    t0 = time.time()
+    plans = batch.plan_frames(specs)
+    log(f"flythrough: {len(plans)} frame plans in {time.time() - t0:.1f}s "
+        f"(ref_len {min(p.view.ref_len for p in plans)}..{max(p.view.ref_len for p in plans)})")
+    return cfg5, plans
+
+
+def run_flythrough(args, torch, prepared):
+    """Config 5: 64 frames of z^3 + k z + c zooming from width 4 to 1e-200,
+    tiles of uneven size, FP32 FloatExp + BLA for every frame.  A step is the
+    whole batch as one HIP graph: orbit uploads, 64 BLA builds, one persistent
+    pixel pass over every frame's tiles, 64 colourings."""
+    from fractura_amd import batch, engine
+
+    cfg5, plans = prepared
+    b = batch.FrameBatch(plans)
+    lut = engine.get_gpu_cmap_lut("inferno", 2048, False, b.dev.torch_device)
+    tuned = batch.autotune(b, reps=3)
+    log(f"autotune: tile {tuned['tile']} blocks {tuned['blocks']} "
+        f"({tuned['ms']:.3f} ms; {len(tuned['candidates'])} candidates)")
+    b.step(lut)
+    b.stats.zero_()
+    b.render(stats=True)
+    torch.cuda.synchronize()
+    st = b.stats.cpu().numpy().view(np.uint64)
+    iters, plain, bla = int(st[0]), int(st[1]), int(st[2])
+    assert iters == b.iterations()
+    glitched = sum(int(o["glitch"].sum().item()) for o in b.outs)
+    # Eager breakdown: the persistent pass alone, by events.
+    timer = Timer(torch)
+    for _ in range(args.steps):
+        with Span(timer):
+            b.render()
+    pass_ms = timer.mean_ms()
+    wall, graph_ms = time_graph(torch, b.capture(lut), args.warmup, args.steps)
+    flops = plain * FLOPS_FEXP_PLAIN + bla * FLOPS_BLA_STEP
+    tflops = flops / (pass_ms * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_batch(plans, args.cpu_seconds)
+    return {
+        "metric": "Giter/s per GPU, FP32+FloatExp perturb vs FP64 at matched count accuracy",
+        "value": iters / wall / 1e9, "unit": "Giter/s", "ms_per_step": wall * 1e3,
+        "dtype": "fp32+i32exp",
+        "config": {"workload": "config5: 64-frame fly-through of z^3+kz+c, width 4 -> 1e-200, "
+                               "one persistent launch + HIP graph",
+                   "frames": len(plans), "pixels": b.pixels(),
+                   "max_iter": [min(p.view.max_iter for p in plans), max(p.view.max_iter for p in plans)],
+                   "k": [cfg5["gen_kr"], cfg5["gen_ki"]], "tile": list(b.tile), "blocks": b.blocks},
+        "iterations_per_step": iters, "executed_steps": {"plain": plain, "bla": bla},
+        "glitched_pixels": glitched,
+        "step_ms": {"persistent_pass": pass_ms, "graph_replay": graph_ms, "wall": wall * 1e3},
+        "autotune": tuned,
+        "roofline": {"bound": "valu", "kernel": "fexp32_batch_kernel<4>", "achieved": tflops,
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / FP32_PEAK_TFLOPS,
+                     "traffic": measured_traffic("fexp32_batch_kernel<4>"), "kernel_ms": pass_ms},
+        "cpu_baseline": cpu,
+    }
+
+
+def cpu_baseline_batch(plans, seconds):
+    """The oracle's FP64-mantissa FloatExp restatement on all host cores over
+    the fly-through's frames in turn (each a centred tile of at most 256x256),
+    until about `seconds` have run."""
+    from oracle import oracle
+
+    model, _ = cpu_info()
+    threads = oracle.load().or_num_threads()
+    total, its, frames = 0.0, 0, 0
+    while total < seconds:
+        for p in plans:
+            v = p.view.copy()
+            re, im, ref_len = p.orbit
+            sub = v.copy()
+            tw, th = min(256, v.width), min(256, v.height)
+            x0, y0 = (v.width - tw) // 2, (v.height - th) // 2
+            sub.width, sub.height = tw, th
+            sub.ref_px, sub.ref_py = v.ref_px - x0, v.ref_py - y0
+            t0 = time.perf_counter()
+            bla = oracle.bla_build(re, im, ref_len, v.fractal_type, p.bla_tol, 16, v.gen_n,
+                                   v.gen_kr, v.gen_ki)
+            m = oracle.render("fexp64", sub, orbit=(re, im), bla=bla, levels=16)
+            total += time.perf_counter() - t0
+            its += m.iterations
+            frames += 1
+            if total >= seconds:
+                break
+    return {"value": its / total / 1e9, "unit": "Giter/s", "cores": threads, "kind": "port",
+            "cpu": model, "sample": f"{frames} fly-through frames in turn, centred tiles of at most "
+                                    f"256x256, FP64-mantissa FloatExp restatement with its BLA table "
+                                    f"(build included), OpenMP {threads} threads, {total:.1f}s"}
+
+
 def run_copy(args, torch):
     """Plain device copy of a known size: the calibration run for the HBM
     byte counters (FETCH_SIZE / WRITE_SIZE) of the profiler."""
@@ -433,7 +537,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["deep", "fp64", "copy"], default="deep")
+    ap.add_argument("--workload", choices=["deep", "fp64", "flythrough", "copy"], default="deep")
     ap.add_argument("--size", type=int, nargs=2, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -444,10 +548,15 @@ def main():
         sys.exit(2)
     import torch
 
+    # The fly-through's host orbits come from a forked pool: before the GPU is touched.
+    prepared = prepare_flythrough(args) if args.workload == "flythrough" else None
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
-    res = {"deep": run_deep, "fp64": run_fp64, "copy": run_copy}[args.workload](args, torch)
+    if prepared is not None:
+        res = run_flythrough(args, torch, prepared)
+    else:
+        res = {"deep": run_deep, "fp64": run_fp64, "copy": run_copy}[args.workload](args, torch)
     line = {"metric": res.pop("metric"), "value": res.pop("value"), "unit": res.pop("unit"),
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": res.pop("ms_per_step"), "higher_is_better": True, "scaling": "weak",

@@ -1,0 +1,241 @@
+"""Many frames in one persistent launch: the FP32 FloatExp + BLA pixel passes of
+a whole batch of frames (a fly-through, or tiles of uneven size and depth)
+run as one launch that takes 256-pixel tiles from a device counter
+(ap_render_fexp32_batch).  The per-frame pipeline around it (orbit upload,
+BLA build, colouring) is plain launches, so the whole batch can be captured
+as one HIP graph and replayed.
+
+The reference renders a video one frame at a time (workers/video_workers.py)
+with one kernel launch per frame (math_core/engine.py:1776-1806); frame
+parameters follow its zoom-in focal tracking (workers/video_workers.py:67-80).
+"""
+import ctypes as C
+import json
+import multiprocessing as mp
+import os
+from decimal import Decimal
+
+import numpy as np
+
+from . import _abi
+from .device import get_device, ptr
+from .engine import capture_graph, plan_view
+
+TILE_SHAPES = ((16, 16), (32, 8), (8, 32), (64, 4))
+
+
+def frame_plan(spec):
+    """Plan one frame (host reference orbit included) on the FP32 FloatExp +
+    BLA path.  spec: dict with center_x, center_y, width (decimal strings),
+    size [W, H], max_iter, and fractal_type / gen_n / gen_kr / gen_ki."""
+    W, H = spec["size"]
+    return plan_view(W, H, int(spec["max_iter"]), Decimal(spec["center_x"]),
+                     Decimal(spec["center_y"]), Decimal(spec["width"]),
+                     fractal_type=int(spec.get("fractal_type", 0)), gen_n=int(spec.get("gen_n", 3)),
+                     gen_kr=float(spec.get("gen_kr", 0.25)), gen_ki=float(spec.get("gen_ki", 1.0)),
+                     bbsa_order=32, bbsa_tol=1e-6, glitch_mode="Pauldelbrot", fexp_mantissa=32,
+                     force_floatexp=True)
+
+
+def plan_frames(specs, processes=None):
+    """Plans of many frames, their host orbits computed in parallel.
+
+    The orbits are CPU-bound big-integer work, one process per frame; the pool
+    forks, so call this before the process touches the GPU."""
+    procs = processes or min(16, os.cpu_count() or 1, len(specs))
+    if procs <= 1 or len(specs) <= 1:
+        return [frame_plan(s) for s in specs]
+    ctx = mp.get_context("fork")
+    with ctx.Pool(procs) as pool:
+        # Deepest frames first: they take longest.
+        order = sorted(range(len(specs)), key=lambda i: -int(specs[i]["max_iter"]))
+        plans = pool.map(frame_plan, [specs[i] for i in order], chunksize=1)
+    out = [None] * len(specs)
+    for i, p in zip(order, plans):
+        out[i] = p
+    return out
+
+
+def flythrough_specs(cfg5):
+    """Frame specs of a config-5 style fly-through (tests/golden/configs.json)."""
+    return [dict(center_x=f["center_x"], center_y=f["center_y"], width=f["width"], size=f["size"],
+                 max_iter=f["max_iter"], fractal_type=4, gen_n=cfg5["gen_n"],
+                 gen_kr=cfg5["gen_kr"], gen_ki=cfg5["gen_ki"]) for f in cfg5["frames"]]
+
+
+class FrameBatch:
+    """Device state of a batch of planned frames and its launches.
+
+    Frames are laid out in the tile sequence slowest first (max_iter x pixels),
+    so the tail of the persistent launch is made of cheap tiles.  Outputs per
+    frame (in the caller's order): counts i32, smooth f64, iter_map f32,
+    glitch u8, rgb u8 (H, W, 3)."""
+
+    def __init__(self, plans, device=0, tile=(16, 16), blocks=0, levels=16):
+        import torch
+
+        self.torch = torch
+        self.dev = get_device(device)
+        self.dev.set_stream(None)
+        td = self.dev.torch_device
+        self.plans = plans
+        self.tile = tuple(tile)
+        self.blocks = int(blocks)
+        self.levels = levels
+        n = len(plans)
+        self.order = sorted(range(n), key=lambda i: -(plans[i].view.max_iter * plans[i].view.width
+                                                       * plans[i].view.height))
+        self.frames = (_abi.ApFrame * n)()
+        self.outs = [None] * n
+        self.host_orbits, self.dev_orbits, self.tables = [], [], []
+        for slot, i in enumerate(self.order):
+            p = plans[i]
+            v = p.view
+            re, im, ref_len = p.orbit
+            if ref_len < 4:
+                raise ValueError(f"frame {i}: reference orbit of {ref_len} iterations is too short for BLA")
+            W, H = v.width, v.height
+            hre = torch.from_numpy(np.ascontiguousarray(re[: ref_len + 1])).pin_memory()
+            him = torch.from_numpy(np.ascontiguousarray(im[: ref_len + 1])).pin_memory()
+            dre, dim = torch.empty_like(hre, device=td), torch.empty_like(him, device=td)
+            table = torch.empty(self.dev.lib.ap_bla_bytes(ref_len, levels), dtype=torch.uint8, device=td)
+            o = dict(counts=torch.zeros((H, W), dtype=torch.int32, device=td),
+                     smooth=torch.zeros((H, W), dtype=torch.float64, device=td),
+                     iter_map=torch.zeros((H, W), dtype=torch.float32, device=td),
+                     glitch=torch.zeros((H, W), dtype=torch.uint8, device=td),
+                     rgb=torch.zeros((H, W, 3), dtype=torch.uint8, device=td))
+            self.outs[i] = o
+            self.host_orbits.append((hre, him))
+            self.dev_orbits.append((dre, dim))
+            self.tables.append(table)
+            f = self.frames[slot]
+            C.pointer(f.view)[0] = v
+            f.view.bla_levels = levels
+            f.view.mode = _abi.AP_MODE_CENTRE
+            f.orbit = _abi.ApOrbit(dre.data_ptr(), dim.data_ptr(), ref_len, 0)
+            f.bla = table.data_ptr()
+            f.out.counts, f.out.smooth = o["counts"].data_ptr(), o["smooth"].data_ptr()
+            f.out.iter_map, f.out.glitch = o["iter_map"].data_ptr(), o["glitch"].data_ptr()
+        self.counter = torch.zeros(1, dtype=torch.int32, device=td)
+        self.stats = torch.zeros(4, dtype=torch.int64, device=td)
+        self.frames_dev = torch.empty(C.sizeof(self.frames), dtype=torch.uint8, device=td)
+        self.set_tile(self.tile)
+
+    def set_tile(self, tile):
+        """Lay the frames out in tiles of tile[0] x tile[1] pixels."""
+        self.tile = tuple(tile)
+        self.total_tiles = self.dev.lib.ap_batch_tiles(C.cast(self.frames, C.c_void_p),
+                                                       len(self.frames), *self.tile)
+        if self.total_tiles <= 0:
+            raise ValueError(f"bad tile shape {tile}")
+        host = np.frombuffer(bytes(self.frames), dtype=np.uint8)
+        self.frames_dev.copy_(self.torch.from_numpy(host.copy()))
+        self.torch.cuda.synchronize()
+
+    # ---- launches (enqueue only: capturable) --------------------------------
+    def upload(self):
+        """Reference orbits from pinned host memory to the device."""
+        for (hre, him), (dre, dim) in zip(self.host_orbits, self.dev_orbits):
+            dre.copy_(hre, non_blocking=True)
+            dim.copy_(him, non_blocking=True)
+
+    def build(self):
+        """Every frame's BLA table, FP32 form, on the device."""
+        for slot, (f, table) in enumerate(zip(self.frames, self.tables)):
+            v = f.view
+            tol = self.plans[self.order[slot]].bla_tol
+            self.dev.call("ap_bla_build", C.byref(f.orbit), v.fractal_type, v.gen_n, v.gen_kr,
+                          v.gen_ki, tol, self.levels, _abi.AP_BLA_FORM_FP32, table.data_ptr())
+
+    def render(self, stats=False):
+        """All frames' pixel passes: one persistent launch."""
+        self.dev.call("ap_render_fexp32_batch", C.cast(self.frames, C.c_void_p), ptr(self.frames_dev),
+                      len(self.frames),
+                      self.tile[0], self.tile[1], self.blocks, ptr(self.counter),
+                      ptr(self.stats) if stats else None)
+
+    def colour(self, lut):
+        for i, o in enumerate(self.outs):
+            v = self.plans[i].view
+            self.dev.call("ap_colorize", C.byref(v), ptr(o["iter_map"]), None, 0, ptr(lut),
+                          ptr(o["rgb"]))
+
+    def step(self, lut):
+        """The whole batch: upload, build, pixel passes, colouring."""
+        self.upload()
+        self.build()
+        self.render()
+        self.colour(lut)
+
+    def capture(self, lut):
+        return capture_graph(self.dev, lambda: self.step(lut))
+
+    def iterations(self):
+        """Sum over every frame's pixels of min(count, max_iter) (cardioid-
+        shortcut pixels count 0), from the device counts."""
+        total = 0
+        for i, o in enumerate(self.outs):
+            c = o["counts"].to(self.torch.int64)
+            total += int(c.clamp(min=0, max=self.plans[i].view.max_iter).sum().item())
+        return total
+
+    def pixels(self):
+        return sum(p.view.width * p.view.height for p in self.plans)
+
+
+def autotune(batch, lut=None, tiles=TILE_SHAPES, blocks_per_cu=(0, 4, 8), reps=3, cache=None):
+    """Pick the tile shape and persistent grid size of a batch by timing its
+    pass (HIP events, median of `reps`), after one warm-up run each.  The
+    result is stored in `cache` (a JSON file keyed by device and batch
+    signature) and reused when the same batch shape comes again."""
+    torch = batch.torch
+    key = json.dumps({"device": batch.dev.arch,
+                      "frames": [[p.view.width, p.view.height, p.view.max_iter] for p in batch.plans]})
+    if cache and os.path.exists(cache):
+        try:
+            with open(cache) as f:
+                got = json.load(f).get(key)
+            if got:
+                batch.set_tile(tuple(got["tile"]))
+                batch.blocks = got["blocks"]
+                return got
+        except (OSError, ValueError):
+            pass
+    num_cus = batch.dev.num_cus
+    batch.upload()
+    batch.build()
+    torch.cuda.synchronize()
+    results = []
+    for tile in tiles:
+        batch.set_tile(tile)
+        for bpc in blocks_per_cu:
+            batch.blocks = bpc * num_cus
+            batch.render()
+            torch.cuda.synchronize()
+            times = []
+            for _ in range(reps):
+                a = torch.cuda.Event(enable_timing=True)
+                b = torch.cuda.Event(enable_timing=True)
+                a.record()
+                batch.render()
+                b.record()
+                torch.cuda.synchronize()
+                times.append(a.elapsed_time(b))
+            results.append({"tile": list(tile), "blocks": batch.blocks, "ms": float(np.median(times))})
+    best = min(results, key=lambda r: r["ms"])
+    batch.set_tile(tuple(best["tile"]))
+    batch.blocks = best["blocks"]
+    out = dict(best, candidates=results)
+    if cache:
+        try:
+            data = {}
+            if os.path.exists(cache):
+                with open(cache) as f:
+                    data = json.load(f)
+            data[key] = out
+            os.makedirs(os.path.dirname(cache) or ".", exist_ok=True)
+            with open(cache, "w") as f:
+                json.dump(data, f)
+        except (OSError, ValueError):
+            pass
+    return out
