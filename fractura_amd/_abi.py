@@ -76,7 +76,15 @@ class ApFrame(C.Structure):
     _fields_ = [
         ("view", ApView), ("orbit", ApOrbit), ("bla", C.c_void_p), ("out", ApOutputs),
         ("tile_begin", C.c_int32), ("tiles_x", C.c_int32), ("tiles", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("colour_begin", C.c_int32),
+    ]
+
+
+class ApBlaJob(C.Structure):
+    _fields_ = [
+        ("orbit", ApOrbit), ("table", C.c_void_p), ("gen_kr", C.c_double), ("gen_ki", C.c_double),
+        ("tol", C.c_double), ("fractal_type", C.c_int32), ("gen_n", C.c_int32),
+        ("low_begin", C.c_int32), ("entry_begin", C.c_int32),
     ]
 
 
@@ -116,6 +124,9 @@ SIGNATURES = {
     "ap_colorize": (C.c_int, [_P, _VIEW, _P, _P, _I, _P, _P]),
     "ap_batch_tiles": (C.c_int32, [_P, _I, _I, _I]),
     "ap_render_fexp32_batch": (C.c_int, [_P, _P, _P, _I, _I, _I, _I, _P, _P]),
+    "ap_colorize_batch": (C.c_int, [_P, _P, _P, _I, _P]),
+    "ap_bla_jobs_layout": (C.c_int64, [_P, _I, _I]),
+    "ap_bla_build_batch": (C.c_int, [_P, _P, _P, _I, _I, _I]),
 }
 
 _lib = None

@@ -116,6 +116,18 @@ class FrameBatch:
             f.bla = table.data_ptr()
             f.out.counts, f.out.smooth = o["counts"].data_ptr(), o["smooth"].data_ptr()
             f.out.iter_map, f.out.glitch = o["iter_map"].data_ptr(), o["glitch"].data_ptr()
+            f.out.rgb = o["rgb"].data_ptr()
+        # One BLA job per frame, in the same order.
+        self.jobs = (_abi.ApBlaJob * n)()
+        for slot, i in enumerate(self.order):
+            f, j, p = self.frames[slot], self.jobs[slot], plans[i]
+            j.orbit = f.orbit
+            j.table = f.bla
+            j.fractal_type, j.gen_n = f.view.fractal_type, f.view.gen_n
+            j.gen_kr, j.gen_ki, j.tol = f.view.gen_kr, f.view.gen_ki, p.bla_tol
+        if self.dev.lib.ap_bla_jobs_layout(C.cast(self.jobs, C.c_void_p), n, levels) <= 0:
+            raise ValueError("BLA job layout failed")
+        self.jobs_dev = torch.from_numpy(np.frombuffer(bytes(self.jobs), dtype=np.uint8).copy()).to(td)
         self.counter = torch.zeros(1, dtype=torch.int32, device=td)
         self.stats = torch.zeros(4, dtype=torch.int64, device=td)
         self.frames_dev = torch.empty(C.sizeof(self.frames), dtype=torch.uint8, device=td)
@@ -140,12 +152,10 @@ class FrameBatch:
             dim.copy_(him, non_blocking=True)
 
     def build(self):
-        """Every frame's BLA table, FP32 form, on the device."""
-        for slot, (f, table) in enumerate(zip(self.frames, self.tables)):
-            v = f.view
-            tol = self.plans[self.order[slot]].bla_tol
-            self.dev.call("ap_bla_build", C.byref(f.orbit), v.fractal_type, v.gen_n, v.gen_kr,
-                          v.gen_ki, tol, self.levels, _abi.AP_BLA_FORM_FP32, table.data_ptr())
+        """Every frame's BLA table, FP32 form: one launch for levels 0..8 of all
+        tables, one per higher level."""
+        self.dev.call("ap_bla_build_batch", C.cast(self.jobs, C.c_void_p), ptr(self.jobs_dev),
+                      len(self.jobs), self.levels, _abi.AP_BLA_FORM_FP32)
 
     def render(self, stats=False):
         """All frames' pixel passes: one persistent launch."""
@@ -155,10 +165,9 @@ class FrameBatch:
                       ptr(self.stats) if stats else None)
 
     def colour(self, lut):
-        for i, o in enumerate(self.outs):
-            v = self.plans[i].view
-            self.dev.call("ap_colorize", C.byref(v), ptr(o["iter_map"]), None, 0, ptr(lut),
-                          ptr(o["rgb"]))
+        """Every frame's palette colouring in one launch."""
+        self.dev.call("ap_colorize_batch", C.cast(self.frames, C.c_void_p), ptr(self.frames_dev),
+                      len(self.frames), ptr(lut))
 
     def step(self, lut):
         """The whole batch: upload, build, pixel passes, colouring."""

@@ -266,7 +266,7 @@ int ap_render_fexp32(ap_ctx* c, const ap_view* v, const ap_orbit* orb, const voi
 
 int32_t ap_batch_tiles(ap_frame* frames, int32_t nframes, int32_t tile_w, int32_t tile_h) {
     if (!frames || nframes <= 0 || tile_w <= 0 || tile_h <= 0) return 0;
-    long long total = 0;
+    long long total = 0, colour = 0;
     for (int i = 0; i < nframes; ++i) {
         ap_frame& f = frames[i];
         f.tiles_x = (f.view.width + tile_w - 1) / tile_w;
@@ -275,8 +275,81 @@ int32_t ap_batch_tiles(ap_frame* frames, int32_t nframes, int32_t tile_w, int32_
         f.tile_begin = (int32_t)total;
         f.tiles = (int32_t)tiles;
         total += tiles;
+        const long long px = (long long)f.view.width * f.view.height;
+        if (colour + (px + ap::kColourPixels - 1) / ap::kColourPixels > INT32_MAX) return 0;
+        f.colour_begin = (int32_t)colour;
+        colour += (px + ap::kColourPixels - 1) / ap::kColourPixels;
     }
     return (int32_t)total;
+}
+
+int ap_colorize_batch(ap_ctx* c, const ap_frame* frames, const ap_frame* frames_dev, int32_t nframes,
+                      const float* lut) {
+    if (!c) return AP_ERR_ARG;
+    if (!frames || !frames_dev || !lut || nframes <= 0)
+        return fail(c, AP_ERR_ARG, "colour batch needs host and device frame arrays and a LUT");
+    long long next = 0;
+    for (int i = 0; i < nframes; ++i) {
+        const ap_frame& f = frames[i];
+        if (!f.out.iter_map || !f.out.rgb || f.view.width <= 0 || f.view.height <= 0)
+            return fail(c, AP_ERR_ARG, "frame %d: colouring needs iter_map and rgb", i);
+        if (((uintptr_t)f.out.iter_map & 15) || ((uintptr_t)f.out.rgb & 15))
+            return fail(c, AP_ERR_ARG, "frame %d: iter_map and rgb must be 16-byte aligned", i);
+        if (f.view.scheme_id < 0 || f.view.scheme_id > 11)
+            return fail(c, AP_ERR_ARG, "frame %d: bad scheme %d", i, f.view.scheme_id);
+        if (f.colour_begin != next) return fail(c, AP_ERR_ARG, "frame %d: colour_begin disagrees (ap_batch_tiles)", i);
+        next += ((long long)f.view.width * f.view.height + ap::kColourPixels - 1) / ap::kColourPixels;
+    }
+    if (next > INT32_MAX) return fail(c, AP_ERR_ARG, "batch exceeds 2^31 colouring blocks");
+    int r = use_device(c);
+    if (r) return r;
+    return check(c, ap::launch_colorize_batch(frames_dev, nframes, (int)next, lut, c->stream),
+                 "colorize batch");
+}
+
+int64_t ap_bla_jobs_layout(ap_bla_job* jobs, int32_t njobs, int32_t levels) {
+    if (!jobs || njobs <= 0 || levels < 2) return 0;
+    long long low = 0, entries = 0;
+    for (int i = 0; i < njobs; ++i) {
+        ap_bla_job& j = jobs[i];
+        if (j.orbit.len < 4 || low > INT32_MAX || entries > INT32_MAX) return 0;
+        j.low_begin = (int32_t)low;
+        j.entry_begin = (int32_t)entries;
+        low += ap::bla_low_blocks(j.orbit.len);
+        entries += j.orbit.len;
+    }
+    if (low > INT32_MAX || entries > INT32_MAX) return 0;
+    return entries;
+}
+
+int ap_bla_build_batch(ap_ctx* c, const ap_bla_job* jobs, const ap_bla_job* jobs_dev, int32_t njobs,
+                       int32_t levels, int32_t forms) {
+    if (!c) return AP_ERR_ARG;
+    if (!jobs || !jobs_dev || njobs <= 0) return fail(c, AP_ERR_ARG, "BLA batch needs host and device job arrays");
+    if (levels < 2 || levels > 31) return fail(c, AP_ERR_ARG, "levels=%d outside [2, 31]", levels);
+    if (forms < 1 || forms > AP_BLA_FORM_ALL) return fail(c, AP_ERR_ARG, "bad BLA forms mask %d", forms);
+    long long low = 0, entries = 0;
+    for (int i = 0; i < njobs; ++i) {
+        const ap_bla_job& j = jobs[i];
+        if (!j.orbit.re || !j.orbit.im || !j.table) return fail(c, AP_ERR_ARG, "job %d: null orbit or table", i);
+        if (j.orbit.len < 4 || !(j.tol > 0.0)) return fail(c, AP_ERR_ARG, "job %d: ref_len=%d or tol too small", i, j.orbit.len);
+        if (j.orbit.len > ap::kMaxBlaRefLen || (long long)j.orbit.len * levels > ap::kMaxBlaEntries)
+            return fail(c, AP_ERR_ARG, "job %d: table exceeds the limits", i);
+        if (j.fractal_type != AP_MANDELBROT && j.fractal_type != AP_JULIA_MANDELBROT &&
+            j.fractal_type != AP_GENERAL_MANDELBROT)
+            return fail(c, AP_ERR_UNSUPPORTED, "job %d: no BLA for fractal type %d", i, j.fractal_type);
+        if (j.fractal_type == AP_GENERAL_MANDELBROT && (j.gen_n < 2 || j.gen_n > 64))
+            return fail(c, AP_ERR_ARG, "job %d: bad gen_n %d", i, j.gen_n);
+        if (j.low_begin != low || j.entry_begin != entries)
+            return fail(c, AP_ERR_ARG, "job %d: low_begin / entry_begin disagree (ap_bla_jobs_layout)", i);
+        low += ap::bla_low_blocks(j.orbit.len);
+        entries += j.orbit.len;
+    }
+    if (low > INT32_MAX || entries > INT32_MAX) return fail(c, AP_ERR_ARG, "BLA batch too large");
+    int r = use_device(c);
+    if (r) return r;
+    return check(c, ap::launch_bla_build_batch(jobs_dev, njobs, levels, forms, (int)low, entries, c->stream),
+                 "bla build batch");
 }
 
 int ap_render_fexp32_batch(ap_ctx* c, const ap_frame* frames, const ap_frame* frames_dev,
@@ -294,14 +367,16 @@ int ap_render_fexp32_batch(ap_ctx* c, const ap_frame* frames, const ap_frame* fr
     long long next = 0;
     for (int i = 0; i < nframes; ++i) {
         const ap_frame& f = frames[i];
-        int r = check_view(c, &f.view, &f.out);
+        ap_outputs maps = f.out;
+        maps.rgb = nullptr;  // the pass writes maps only; ap_colorize_batch colours
+        int r = check_view(c, &f.view, &maps);
         if (r) return r;
         if ((r = check_orbit(c, &f.view, &f.orbit, f.bla))) return r;
         if (f.view.fractal_type != type) return fail(c, AP_ERR_ARG, "frame %d: mixed fractal types", i);
         if (f.view.mode != AP_MODE_CENTRE || f.view.factor != 1 || f.view.rebase_pass != 0)
             return fail(c, AP_ERR_UNSUPPORTED, "frame %d: the batched pass takes centre samples only", i);
-        if (f.out.rgb || f.out.queue || f.out.mask || f.out.stats)
-            return fail(c, AP_ERR_ARG, "frame %d: no rgb, queue, mask or per-frame stats in a batch", i);
+        if (f.out.queue || f.out.mask || f.out.stats)
+            return fail(c, AP_ERR_ARG, "frame %d: no queue, mask or per-frame stats in a batch", i);
         if (!f.bla || f.view.bla_levels < 2 || f.orbit.len < 4)
             return fail(c, AP_ERR_ARG, "frame %d: the batched pass needs a BLA table", i);
         if (f.view.e_scale > 30000 || f.view.e_scale < -30000)

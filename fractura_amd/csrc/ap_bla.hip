@@ -33,7 +33,8 @@ struct BlaMut {
     int state_lo;  // levels [state_lo, levels - 1) are read back by the merges
 };
 
-static BlaMut bla_mut(void* table, int ref_len, int levels, int forms, int state_lo) {
+__host__ __device__ inline BlaMut bla_mut(void* table, int ref_len, int levels, int forms,
+                                          int state_lo) {
     BlaLayout L = bla_layout(ref_len, levels);
     char* p = (char*)table;
     BlaMut b;
@@ -179,18 +180,16 @@ constexpr int kLowSpan = kLowOut + (1 << kLowTop) - 1;
 constexpr int kLowThreads = 256;
 constexpr int kLowPer = (kLowSpan + kLowThreads - 1) / kLowThreads;
 
-// Levels 0..top (top <= kLowTop) for k in [k0, k0 + kLowOut).  Level d is
-// needed on [k0, k0 + kLowOut + 2^top - 2^d): each merge reads its own index
-// and the one 2^(d-1) to the right, so every entry of level d-1 is read
-// before any of level d is written (one barrier between the phases).
-__global__ __launch_bounds__(kLowThreads) void bla_low_kernel(BlaMut t,
-                                                              const double* __restrict__ zr,
-                                                              const double* __restrict__ zi,
-                                                              int type, int n, double kr,
-                                                              double ki, double tol, int top) {
-    __shared__ double4 sc[kLowSpan];
-    __shared__ double sr[kLowSpan];
-    const int k0 = blockIdx.x * kLowOut;
+// Levels 0..top (top <= kLowTop) for k in [k0, k0 + kLowOut), k0 = blk *
+// kLowOut.  Level d is needed on [k0, k0 + kLowOut + 2^top - 2^d): each merge
+// reads its own index and the one 2^(d-1) to the right, so every entry of
+// level d-1 is read before any of level d is written (one barrier between the
+// phases).  sc/sr: the block's LDS, kLowSpan entries each.
+__device__ inline void bla_low_block(const BlaMut& t, const double* __restrict__ zr,
+                                     const double* __restrict__ zi, int type, int n, double kr,
+                                     double ki, double tol, int top, int blk, double4* sc,
+                                     double* sr) {
+    const int k0 = blk * kLowOut;
     const int ref_len = t.ref_len;
     const int span0 = kLowOut + (1 << top) - 1;
     for (int i = threadIdx.x; i < span0; i += kLowThreads) {
@@ -233,10 +232,43 @@ __global__ __launch_bounds__(kLowThreads) void bla_low_kernel(BlaMut t,
     }
 }
 
-// Level d from level d-1 over all k: lanes k and k+1 read adjacent words.
-__global__ __launch_bounds__(256) void bla_merge_kernel(BlaMut t, int d) {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= t.ref_len) return;
+__global__ __launch_bounds__(kLowThreads) void bla_low_kernel(BlaMut t,
+                                                              const double* __restrict__ zr,
+                                                              const double* __restrict__ zi,
+                                                              int type, int n, double kr,
+                                                              double ki, double tol, int top) {
+    __shared__ double4 sc[kLowSpan];
+    __shared__ double sr[kLowSpan];
+    bla_low_block(t, zr, zi, type, n, kr, ki, tol, top, blockIdx.x, sc, sr);
+}
+
+// Batched tables: block b belongs to the last job whose low_begin <= b
+// (a block-uniform binary search over the job array).
+__device__ inline int job_of(const ap_bla_job* jobs, int njobs, long long x, bool entries) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((entries ? jobs[mid].entry_begin : jobs[mid].low_begin) <= x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kLowThreads) void bla_low_batch_kernel(const ap_bla_job* __restrict__ jobs,
+                                                                    int njobs, int levels, int forms,
+                                                                    int top) {
+    __shared__ double4 sc[kLowSpan];
+    __shared__ double sr[kLowSpan];
+    const ap_bla_job& J = jobs[job_of(jobs, njobs, blockIdx.x, false)];
+    const BlaMut t = bla_mut(J.table, J.orbit.len, levels, forms, top);
+    bla_low_block(t, J.orbit.re, J.orbit.im, J.fractal_type, J.gen_n, J.gen_kr, J.gen_ki, J.tol,
+                  top, blockIdx.x - J.low_begin, sc, sr);
+}
+
+// Entry k of level d from level d-1: lanes k and k+1 read adjacent words.
+__device__ inline void bla_merge_entry(const BlaMut& t, int d, int k) {
     const size_t N = (size_t)t.ref_len;
     const int step = 1 << (d - 1);
     BlaEntry out = bla_zero();
@@ -251,6 +283,22 @@ __global__ __launch_bounds__(256) void bla_merge_kernel(BlaMut t, int d) {
         out = bla_merge(e1, e2);
     }
     store_entry(t, k, d, out);
+}
+
+__global__ __launch_bounds__(256) void bla_merge_kernel(BlaMut t, int d) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < t.ref_len) bla_merge_entry(t, d, k);
+}
+
+// Level d of every batched table: entry g of the batch is entry
+// g - entry_begin of its job.
+__global__ __launch_bounds__(256) void bla_merge_batch_kernel(const ap_bla_job* __restrict__ jobs,
+                                                              int njobs, int levels, int forms,
+                                                              int top, int d, long long total) {
+    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (g >= total) return;
+    const ap_bla_job& J = jobs[job_of(jobs, njobs, g, true)];
+    bla_merge_entry(bla_mut(J.table, J.orbit.len, levels, forms, top), d, (int)(g - J.entry_begin));
 }
 
 __global__ __launch_bounds__(256) void bla_export_kernel(BlaView t, double* __restrict__ out) {
@@ -281,6 +329,20 @@ hipError_t launch_bla_build(const ap_orbit& orb, int type, int n, double kr, dou
         hipLaunchKernelGGL(bla_merge_kernel, dim3(blocks), dim3(256), 0, s, t, d);
     return hipGetLastError();
 }
+
+hipError_t launch_bla_build_batch(const ap_bla_job* jobs_dev, int njobs, int levels, int forms,
+                                  int low_blocks, long long entries, hipStream_t s) {
+    const int top = levels - 1 < kLowTop ? levels - 1 : kLowTop;
+    hipLaunchKernelGGL(bla_low_batch_kernel, dim3(low_blocks), dim3(kLowThreads), 0, s, jobs_dev,
+                       njobs, levels, forms, top);
+    const unsigned blocks = (unsigned)((entries + 255) / 256);
+    for (int d = top + 1; d < levels; ++d)
+        hipLaunchKernelGGL(bla_merge_batch_kernel, dim3(blocks), dim3(256), 0, s, jobs_dev, njobs,
+                           levels, forms, top, d, entries);
+    return hipGetLastError();
+}
+
+int bla_low_blocks(int ref_len) { return ref_len / kLowOut + 1; }
 
 hipError_t launch_bla_export(const void* table, int ref_len, int levels, double* out,
                              hipStream_t s) {

@@ -335,13 +335,12 @@ __device__ inline void colour_group(const ap_view& v, const ap_outputs& o, float
     }
 }
 
-__global__ __launch_bounds__(kColThreads) void colorize_kernel(ap_view v, const float* __restrict__ it,
-                                                               const float* __restrict__ cdf,
-                                                               int num_bins,
-                                                               const float* __restrict__ lut,
-                                                               void* __restrict__ rgb, long long n) {
-    __shared__ uint4 stage[kColPx * 6 / 16];
-    const long long pb = (long long)blockIdx.x * kColPx;
+// Block blk of a map of n pixels: pixels [blk * kColPx, (blk + 1) * kColPx).
+__device__ __attribute__((always_inline)) inline void colour_block(const ap_view& v, const float* __restrict__ it,
+                                    const float* __restrict__ cdf, int num_bins,
+                                    const float* __restrict__ lut, void* __restrict__ rgb,
+                                    long long n, long long blk, uint4* stage) {
+    const long long pb = blk * kColPx;
     ap_outputs o = {};
     o.cdf = cdf;
     o.num_bins = num_bins;
@@ -374,6 +373,44 @@ __global__ __launch_bounds__(kColThreads) void colorize_kernel(ap_view v, const 
     const int words = v.is_hdr ? kColPx * 6 / 16 : kColPx * 3 / 16;
     uint4* dst = (uint4*)((char*)rgb + pb * (v.is_hdr ? 6 : 3));
     for (int i = threadIdx.x; i < words; i += kColThreads) dst[i] = stage[i];
+}
+
+__global__ __launch_bounds__(kColThreads) void colorize_kernel(ap_view v, const float* __restrict__ it,
+                                                               const float* __restrict__ cdf,
+                                                               int num_bins,
+                                                               const float* __restrict__ lut,
+                                                               void* __restrict__ rgb, long long n) {
+    __shared__ uint4 stage[kColPx * 6 / 16];
+    colour_block(v, it, cdf, num_bins, lut, rgb, n, blockIdx.x, stage);
+}
+
+// Every frame of a batch: block b colours block b - colour_begin of the last
+// frame whose colour_begin <= b (a block-uniform binary search).
+__global__ __launch_bounds__(kColThreads) void colorize_batch_kernel(const ap_frame* __restrict__ frames,
+                                                                     int nframes,
+                                                                     const float* __restrict__ lut) {
+    __shared__ uint4 stage[kColPx * 6 / 16];
+    int lo = 0, hi = nframes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (frames[mid].colour_begin <= (int)blockIdx.x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const ap_frame& F = frames[lo];
+    colour_block(F.view, F.out.iter_map, nullptr, 0, lut, F.out.rgb,
+                 (long long)F.view.width * F.view.height, (long long)blockIdx.x - F.colour_begin,
+                 stage);
+}
+
+static_assert(kColPx == kColourPixels, "colouring block size");
+
+hipError_t launch_colorize_batch(const ap_frame* frames_dev, int nframes, int blocks,
+                                 const float* lut, hipStream_t s) {
+    hipLaunchKernelGGL(colorize_batch_kernel, dim3(blocks), dim3(kColThreads), 0, s, frames_dev,
+                       nframes, lut);
+    return hipGetLastError();
 }
 
 hipError_t launch_colorize(const ap_view& v, const float* it, const float* cdf, int num_bins,

@@ -29,6 +29,9 @@ constexpr long long kMaxBlaEntries = 1LL << 27;
 size_t bla_bytes(int ref_len, int levels);
 hipError_t launch_bla_build(const ap_orbit& orb, int type, int n, double kr, double ki,
                             double tol, int levels, int forms, void* table, hipStream_t s);
+hipError_t launch_bla_build_batch(const ap_bla_job* jobs_dev, int njobs, int levels, int forms,
+                                  int low_blocks, long long entries, hipStream_t s);
+int bla_low_blocks(int ref_len);  // blocks of the levels-0..8 kernel for one table
 hipError_t launch_bla_export(const void* table, int ref_len, int levels, double* out,
                              hipStream_t s);
 
@@ -44,5 +47,8 @@ hipError_t launch_hist_build(const float* iter_map, int n, int max_iter, int num
 hipError_t launch_hist_cdf(const int* hist, int num_bins, float* cdf, hipStream_t s);
 hipError_t launch_colorize(const ap_view& v, const float* iter_map, const float* cdf,
                            int num_bins, const float* lut, void* rgb, hipStream_t s);
+hipError_t launch_colorize_batch(const ap_frame* frames_dev, int nframes, int blocks,
+                                 const float* lut, hipStream_t s);
+constexpr int kColourPixels = 4096;  // pixels per colouring block
 
 }  // namespace ap

@@ -139,12 +139,23 @@ typedef struct ap_frame {
     ap_view view;         /* mode 0 (centre sample), factor 1               */
     ap_orbit orbit;
     const void *bla;      /* BLA table, required                            */
-    ap_outputs out;       /* counts / smooth / iter_map / glitch; no rgb    */
+    ap_outputs out;       /* counts / smooth / iter_map / glitch; rgb is
+                             written by ap_colorize_batch, not by the pass  */
     int32_t tile_begin;   /* index of the frame's first tile                */
     int32_t tiles_x;      /* tiles per row = ceil(width / tile_w)           */
     int32_t tiles;        /* tiles of the frame                             */
-    int32_t reserved0;
+    int32_t colour_begin; /* first colouring block (4096 pixels each)       */
 } ap_frame;
+
+/* One table of a batched BLA build (ap_bla_build_batch). */
+typedef struct ap_bla_job {
+    ap_orbit orbit;
+    void *table;          /* ap_bla_bytes(orbit.len, levels) bytes          */
+    double gen_kr, gen_ki, tol;
+    int32_t fractal_type, gen_n;
+    int32_t low_begin;    /* filled by ap_bla_jobs_layout                   */
+    int32_t entry_begin;  /* filled by ap_bla_jobs_layout                   */
+} ap_bla_job;
 
 /* ---- context, stream and memory ------------------------------------------
  * Replaces init_primary_cuda_device (math_core/cuda_kernels.py:11-30), the
@@ -201,13 +212,24 @@ int ap_render_fexp32(ap_ctx *ctx, const ap_view *view, const ap_orbit *orbit,
  * fast ones share the device without a tail per frame.  The frames array is
  * read by the device (frames_dev) and checked on the host (frames, the same
  * contents); every frame has the same fractal type (0, 2 or 4).
- * ap_batch_tiles fills tile_begin / tiles_x / tiles in array order and
- * returns the total (callers put the slowest frames first).  *counter is
- * device scratch of one int32 (the library resets it on the stream). */
+ * ap_batch_tiles fills tile_begin / tiles_x / tiles / colour_begin in array
+ * order and returns the total tile count (callers put the slowest frames
+ * first).  *counter is device scratch of one int32 (the library resets it on
+ * the stream).  ap_colorize_batch colours every frame's iter_map into its rgb
+ * (palette scheme of its view) in one launch, as ap_colorize does per frame.
+ * ap_bla_build_batch builds many tables (same levels and forms) with one
+ * launch for levels 0..8 and one per higher level, as ap_bla_build does per
+ * table; ap_bla_jobs_layout fills low_begin / entry_begin and returns the
+ * number of entries. */
 int32_t ap_batch_tiles(ap_frame *frames, int32_t nframes, int32_t tile_w, int32_t tile_h);
 int ap_render_fexp32_batch(ap_ctx *ctx, const ap_frame *frames, const ap_frame *frames_dev,
                            int32_t nframes, int32_t tile_w, int32_t tile_h, int32_t blocks,
                            int32_t *counter, unsigned long long *stats);
+int ap_colorize_batch(ap_ctx *ctx, const ap_frame *frames, const ap_frame *frames_dev,
+                      int32_t nframes, const float *lut);
+int64_t ap_bla_jobs_layout(ap_bla_job *jobs, int32_t njobs, int32_t levels);
+int ap_bla_build_batch(ap_ctx *ctx, const ap_bla_job *jobs, const ap_bla_job *jobs_dev,
+                       int32_t njobs, int32_t levels, int32_t forms);
 
 /* ---- chained BLA table -----------------------------------------------------
  * Replaces _jit_build_bla_table (math_core/high_precision.py:1100-1238) and

@@ -16,7 +16,7 @@ def declared_symbols():
 
 
 STRUCTS = (("ap_view", _abi.ApView), ("ap_outputs", _abi.ApOutputs), ("ap_orbit", _abi.ApOrbit),
-           ("ap_frame", _abi.ApFrame))
+           ("ap_frame", _abi.ApFrame), ("ap_bla_job", _abi.ApBlaJob))
 
 
 def test_header_declares_the_path_entry_points():
