@@ -6,15 +6,17 @@ recolor_frame, RenderParameters, ...); the kernels live in
 libapeiron_hip.so (HIP, gfx950) behind the C ABI of include/apeiron_hip.h.
 """
 from .config import RenderParameters, compute_floatexp_scale, compute_required_prec_bits
-from .engine import (clear_scratch_buffer_pool, compute_dynamic_max_iter, get_scratch_device_array,
-                     plan_view, recolor_frame, render_frame, render_frame_cpu, render_frame_ssaa,
-                     render_view, select_engine)
+from .engine import (capture_graph, clear_scratch_buffer_pool, compute_dynamic_max_iter,
+                     get_adaptive_max_iter_from_probe, get_scratch_device_array, plan_view,
+                     probe_adaptive_max_iter, recolor_frame, render_frame, render_frame_cpu,
+                     render_frame_ssaa, render_view, select_engine)
 from .high_precision import (cache_reference_orbit, compute_highprec_escape_scalar,
                              compute_single_highprec_orbit, find_optimal_reference_orbit, to_dd)
 
 __all__ = [
     "RenderParameters", "render_frame", "render_frame_ssaa", "render_frame_cpu", "recolor_frame",
     "render_view", "plan_view", "select_engine", "compute_dynamic_max_iter",
+    "probe_adaptive_max_iter", "get_adaptive_max_iter_from_probe", "capture_graph",
     "get_scratch_device_array", "clear_scratch_buffer_pool", "to_dd",
     "compute_single_highprec_orbit", "compute_highprec_escape_scalar",
     "find_optimal_reference_orbit", "cache_reference_orbit", "compute_floatexp_scale",

@@ -565,18 +565,63 @@ def recolor_frame(params, d_iter_buf=None, h_iter_buf=None, device=0):
         return out
 
 
+def get_adaptive_max_iter_from_probe(iters_array, base_iter):
+    """max_iter from a probe map (math_core/engine.py:1051-1060): 1.35 x the
+    98th percentile or 1.2 x the maximum of the escaped samples, whichever is
+    larger, at least 100; the base value when nothing escaped."""
+    a = np.asarray(iters_array)
+    escaped = a[a >= 0.0]
+    if escaped.size == 0:
+        return int(base_iter)
+    max_escaped = float(np.max(escaped))
+    p98 = float(np.percentile(escaped, 98))
+    target = int(max(p98 * 1.35, max_escaped * 1.20))
+    return min(GLOBAL_MAX_ITER, max(100, target))
+
+
+def probe_adaptive_max_iter(center_x_dec, center_y_dec, plot_width_dec, base_iter, target_w=128,
+                            target_h=72, fractal_type=0, julia_cx=0.0, julia_cy=0.0, gen_n=3,
+                            gen_kr=0.25, gen_ki=1.0, precision_mode="1e-300 (Perturbation)",
+                            bbsa_tol=1e-4, bbsa_order=4, compute_device="GPU", cpu_threads=2,
+                            device=0, **_ignored):
+    """A 128x72 probe pass at max_iter = max(1000, 2 base), clamped, through the
+    same engine a frame of this view would use, then
+    get_adaptive_max_iter_from_probe (math_core/engine.py:1063-1281).  The
+    reference's probe kernels use its BBSA series for the perturbation modes;
+    this probe takes the engine's own BLA instead, the same iteration counts up
+    to the approximation tolerance."""
+    import torch
+
+    if str(compute_device).upper() == "CPU":
+        raise NotImplementedError("fractura_amd renders on the GPU only; use the reference CPU path")
+    search_max = min(GLOBAL_MAX_ITER, max(1000, int(base_iter * 2)))
+    plan = plan_view(int(target_w), int(target_h), search_max, center_x_dec, center_y_dec,
+                     plot_width_dec, fractal_type=fractal_type, julia_cx=julia_cx,
+                     julia_cy=julia_cy, gen_n=gen_n, gen_kr=gen_kr, gen_ki=gen_ki,
+                     precision_mode=precision_mode, bbsa_tol=bbsa_tol, bbsa_order=bbsa_order)
+    fr = render_view(plan, "inferno", device=device, rebase=False, want_counts=False,
+                     want_smooth=False, stats=False)
+    torch.cuda.synchronize()
+    return get_adaptive_max_iter_from_probe(fr.iter_map.cpu().numpy(), base_iter)
+
+
 def compute_dynamic_max_iter(base_max_iter, plot_width_dec, dynamic_mode,
                              center_x_dec=Decimal("0.0"), center_y_dec=Decimal("0.0"),
                              **probe_kwargs):
-    """Zoom-dependent max_iter (math_core/engine.py:1284-1323).  The adaptive
-    probe mode is not built here and keeps the base value, as the reference
-    does when its probe fails."""
+    """Zoom-dependent max_iter (math_core/engine.py:1284-1323); the adaptive
+    mode runs the probe pass and keeps the base value if the probe fails, as
+    the reference does."""
     clamped = min(GLOBAL_MAX_ITER, max(1, int(base_max_iter)))
     if dynamic_mode == "Off" or not dynamic_mode or base_max_iter <= 0:
         return clamped
     mode = str(dynamic_mode).lower()
     if "adaptive" in mode:
-        return clamped
+        try:
+            return probe_adaptive_max_iter(center_x_dec=center_x_dec, center_y_dec=center_y_dec,
+                                           plot_width_dec=plot_width_dec, base_iter=clamped,
+                                           **probe_kwargs)
+        except Exception:  # the reference falls back to the base value on any failure
+            return clamped
     try:
         p = Decimal(str(plot_width_dec))
         if p <= 0 or p >= Decimal("4.0"):
