@@ -29,6 +29,10 @@ AP_BLA_FORM_FP64 = 1
 AP_BLA_FORM_FP32 = 2
 AP_BLA_FORM_ALL = 3
 
+AP_LIMBS_GPU34 = 0
+AP_LIMBS_HIGHPREC = 1
+AP_MAX_LIMBS = 256
+
 
 class ApView(C.Structure):
     _fields_ = [
@@ -110,6 +114,7 @@ SIGNATURES = {
     "ap_device_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
     "ap_render_fp64": (C.c_int, [_P, _VIEW, _OUT]),
     "ap_render_dd": (C.c_int, [_P, _VIEW, _OUT]),
+    "ap_render_limbs": (C.c_int, [_P, _VIEW, _I, _I, _P, _I, _P, _P, _P, _P, _OUT]),
     "ap_render_pert64": (C.c_int, [_P, _VIEW, _ORB, _P, _OUT]),
     "ap_render_fexp64": (C.c_int, [_P, _VIEW, _ORB, _P, _OUT]),
     "ap_render_fexp32": (C.c_int, [_P, _VIEW, _ORB, _P, _OUT]),

@@ -3,6 +3,7 @@
 // launch the kernels of ap_direct.hip, ap_pert.hip, ap_bla.hip and
 // ap_color.hip.  Nothing here throws; every failure is an int status with the
 // text kept in the context.
+#include <math.h>
 #include <stdarg.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -220,6 +221,36 @@ int ap_render_dd(ap_ctx* c, const ap_view* v, const ap_outputs* o) {
     if (r) return r;
     if ((r = use_device(c))) return r;
     return check(c, ap::launch_dd(*v, *o, c->stream), "dd pass");
+}
+
+int ap_render_limbs(ap_ctx* c, const ap_view* v, int32_t semantics, int32_t limbs,
+                    const int32_t* queue, int32_t n, const uint32_t* coords, const int32_t* signs,
+                    const uint32_t* params, const int32_t* param_signs, const ap_outputs* o) {
+    if (!c) return AP_ERR_ARG;
+    if (!v || !o) return fail(c, AP_ERR_ARG, "null view or outputs");
+    if (semantics != AP_LIMBS_GPU34 && semantics != AP_LIMBS_HIGHPREC)
+        return fail(c, AP_ERR_ARG, "bad limb semantics %d", semantics);
+    if (limbs < 2 || limbs > AP_MAX_LIMBS)
+        return fail(c, AP_ERR_ARG, "limbs=%d outside [2, %d]", limbs, AP_MAX_LIMBS);
+    if (n < 0) return fail(c, AP_ERR_ARG, "negative pixel count");
+    if (n > 0 && (!queue || !coords || !signs)) return fail(c, AP_ERR_ARG, "null pixel list");
+    if (!params || !param_signs) return fail(c, AP_ERR_ARG, "null parameter limbs");
+    if (v->width <= 0 || v->height <= 0) return fail(c, AP_ERR_ARG, "bad size %dx%d", v->width, v->height);
+    if (v->max_iter < 0) return fail(c, AP_ERR_ARG, "negative max_iter");
+    if (v->fractal_type < 0 || v->fractal_type > 4) return fail(c, AP_ERR_ARG, "bad fractal type %d", v->fractal_type);
+    if (v->fractal_type == AP_GENERAL_MANDELBROT && (v->gen_n < 1 || v->gen_n > 64))
+        return fail(c, AP_ERR_ARG, "bad gen_n %d", v->gen_n);
+    if (v->fractal_type == AP_GENERAL_MANDELBROT && semantics == AP_LIMBS_HIGHPREC &&
+        !(fabs(v->gen_kr) <= 1024.0 && fabs(v->gen_ki) <= 1024.0))
+        return fail(c, AP_ERR_ARG, "|gen_kr|, |gen_ki| must be <= 1024 for the limb evaluator");
+    if (v->scheme_id < 0 || v->scheme_id > 11) return fail(c, AP_ERR_ARG, "bad scheme %d", v->scheme_id);
+    if (o->rgb && !o->lut) return fail(c, AP_ERR_ARG, "rgb output needs a LUT");
+    if (o->cdf && o->num_bins < 2) return fail(c, AP_ERR_ARG, "cdf needs num_bins >= 2");
+    int r;
+    if ((r = use_device(c))) return r;
+    return check(c, ap::launch_limbs(semantics, limbs, *v, queue, n, coords, signs, params,
+                                     param_signs, *o, c->stream),
+                 "limb pass");
 }
 
 int ap_render_pert64(ap_ctx* c, const ap_view* v, const ap_orbit* orb, const void* bla,

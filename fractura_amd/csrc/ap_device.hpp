@@ -165,11 +165,18 @@ __device__ inline double tri_wave(double phase) {
     return 1.0 - fabs((phase < 0.0 ? phase + 2.0 : phase) - 1.0);
 }
 
+// Square root of the sample in its own precision: Numba compiles
+// cpu_palette_norm once per argument type, so math.sqrt of a float32 sample
+// is a float32 square root and of a float64 sample (the arbitrary-precision
+// fallback, math_core/engine.py:256-264) a float64 one.
+__device__ inline double sample_root(float s) { return (double)sqrtf(s); }
+__device__ inline double sample_root(double s) { return sqrt(s); }
+
 // Palette transfer functions 0..10 (and 11 and above fall back to 0), as in
-// cpu_palette_norm (math_core/cpu_kernels.py:14-78).  The square roots of the
-// float32 sample are float32 operations there (Numba types math.sqrt of a
-// float32 as float32); everything else is double.
-__device__ inline double palette_norm(float s, int scheme, double off, double density,
+// cpu_palette_norm (math_core/cpu_kernels.py:14-78); everything but the
+// square root of the sample is double.
+template <class S>
+__device__ inline double palette_norm(S s, int scheme, double off, double density,
                                       double contrast) {
     double dens = density <= 0.0 ? 1.0 : density;
     double cont = contrast <= 0.0 ? 1.0 : contrast;
@@ -180,7 +187,7 @@ __device__ inline double palette_norm(float s, int scheme, double off, double de
         return tri_wave(ph);
     }
     case 2: {
-        double ph = pymod((double)sqrtf(s) * (0.125 * dens) + off * 2.0, 2.0);
+        double ph = pymod(sample_root(s) * (0.125 * dens) + off * 2.0, 2.0);
         double t = tri_wave(ph);
         double k = 4.0 * cont;
         double td = tanh(0.5 * k);
@@ -188,7 +195,7 @@ __device__ inline double palette_norm(float s, int scheme, double off, double de
         return 0.5 + tanh((t - 0.5) * k) * scale;
     }
     case 3: {
-        double ph = pymod((double)sqrtf(s) * (0.125 * dens) + off * 2.0, 2.0);
+        double ph = pymod(sample_root(s) * (0.125 * dens) + off * 2.0, 2.0);
         double t = tri_wave(ph);
         double k = 3.0 * cont;
         double ed = exp(k) - 1.0;
@@ -196,7 +203,7 @@ __device__ inline double palette_norm(float s, int scheme, double off, double de
         return (exp(k * t) - 1.0) * scale;
     }
     case 4: {
-        double ph = pymod((double)sqrtf(s) * (0.5 * dens) + off * 2.0, 2.0);
+        double ph = pymod(sample_root(s) * (0.5 * dens) + off * 2.0, 2.0);
         double t = tri_wave(ph);
         return cont != 1.0 ? pow(t, cont) : t;
     }
@@ -220,7 +227,7 @@ __device__ inline double palette_norm(float s, int scheme, double off, double de
         return 0.5 + scale * wave;
     }
     case 8: {
-        double ph = pymod((double)sqrtf(s) * (0.1 * dens) + off * 2.0, 2.0);
+        double ph = pymod(sample_root(s) * (0.1 * dens) + off * 2.0, 2.0);
         double t = tri_wave(ph);
         if (cont != 1.0) t = pow(t, cont);
         double t2 = t * t;
@@ -236,7 +243,7 @@ __device__ inline double palette_norm(float s, int scheme, double off, double de
         return raw >= 0.0 ? raw : raw + 1.0;
     }
     default: {  // 0 and > 10
-        double ph = pymod((double)sqrtf(s) * (0.125 * dens) + off * 2.0, 2.0);
+        double ph = pymod(sample_root(s) * (0.125 * dens) + off * 2.0, 2.0);
         double t = tri_wave(ph);
         return cont != 1.0 ? pow(t, cont) : t;
     }
@@ -246,7 +253,8 @@ __device__ inline double palette_norm(float s, int scheme, double off, double de
 // Histogram-equalised transfer (sample_cdf_norm, math_core/cuda_kernels.py:
 // 236-259): CDF lerp at s/max_iter*(bins-1), the difference of the two CDF
 // entries taken in float32 as the reference does.
-__device__ inline double cdf_norm(float s, int max_iter, const float* cdf, int num_bins,
+template <class S>
+__device__ inline double cdf_norm(S s, int max_iter, const float* cdf, int num_bins,
                                   double off, double density, double contrast) {
     double inv_max = 1.0 / fmax(1.0, (double)max_iter);
     double pos = ((double)s * inv_max) * (double)(num_bins - 1);
@@ -300,7 +308,8 @@ __device__ inline void store_rgb(void* rgb, int is_hdr, long long idx, double r,
 
 // Colour of one sample for the RGB accumulators: palette scheme, or the
 // histogram CDF when one is given.
-__device__ inline void sample_colour(const ap_view& v, const ap_outputs& o, float s, double& r,
+template <class S>
+__device__ inline void sample_colour(const ap_view& v, const ap_outputs& o, S s, double& r,
                                      double& g, double& b) {
     double norm = o.cdf ? cdf_norm(s, v.max_iter, o.cdf, o.num_bins, v.palette_offset,
                                    v.color_density, v.color_contrast)

@@ -17,6 +17,11 @@ hipError_t launch_pert64(const ap_view& v, const ap_orbit& orb, const void* bla,
 hipError_t launch_fexp(int mant, const ap_view& v, const ap_orbit& orb, const void* bla,
                        const ap_outputs& o, void* scratch_f32_orbit, hipStream_t s);
 
+// Multi-limb exhaustive fallback (semantics AP_LIMBS_*), one pixel per wave.
+hipError_t launch_limbs(int semantics, int limbs, const ap_view& v, const int32_t* queue, int n,
+                        const uint32_t* coords, const int32_t* signs, const uint32_t* params,
+                        const int32_t* psigns, const ap_outputs& o, hipStream_t s);
+
 // Batched FP32 FloatExp frames in one persistent launch (types 0, 2, 4).
 hipError_t launch_fexp32_batch(int type, const ap_frame* frames_dev, int nframes, int total,
                                int tile_w, int tile_h, int blocks, int* counter,

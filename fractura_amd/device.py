@@ -46,6 +46,7 @@ class Device:
             raise _abi.ApError(st, "ap_init", f"device {self.index}")
         self.handle = h
         self.lock = threading.RLock()
+        self.stream_handle = 0  # raw handle of the stream launches go to
         self.torch_device = torch.device("cuda", self.index)
         cus, clk = C.c_int(), C.c_int()
         name = C.create_string_buffer(256)
@@ -70,6 +71,7 @@ class Device:
         st = self.lib.ap_set_stream(self.handle, C.c_void_p(handle))
         if st != _abi.AP_OK:
             raise _abi.ApError(st, "ap_set_stream", "")
+        self.stream_handle = handle
         return handle
 
 

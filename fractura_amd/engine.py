@@ -191,6 +191,8 @@ class Frame:
     plan: ViewPlan
     rebase_passes: int = 0
     glitch_counts: list = field(default_factory=list)
+    exhaustive_pixels: int = 0   # Strict Metric: pixels re-evaluated directly
+    exhaustive_tier: int = -1    # their precision tier (0 FP64 .. 3 limbs)
 
     def host_stats(self):
         s = self.stats.cpu().numpy().view(np.uint64)
@@ -462,6 +464,26 @@ def _rebase_loop(dev, plan, v, frame, lut, cdf, slot, glitch_mode):
                       mask=gin, queue=queue, queue_count=qn, stats=frame.stats)
         _launch(dev, plan.engine, rv, orbit, o)
     frame.rebase_passes = passes
+    if exhaustive:
+        # Strict Metric: every pixel still glitched is evaluated again
+        # directly at the width's precision tier (math_core/engine.py:2108-2128).
+        dev.call("ap_compact", ptr(frame.glitch), n, ptr(queue), ptr(qn))
+        count = int(qn.cpu().item())
+        frame.exhaustive_pixels = count
+        if count:
+            idx = queue[:count].cpu().numpy().astype(np.int64)
+            from .exhaustive import eval_glitched_pixels_fallback_cuda
+
+            frame.exhaustive_tier = eval_glitched_pixels_fallback_cuda(
+                frame.rgb, frame.glitch, lut, np.stack([idx // W, idx % W], axis=1), W, H,
+                v.max_iter, plan.center_x_dec, plan.center_y_dec, plan.plot_width_dec,
+                plan.plot_height_dec, v.scheme_id, v.fractal_type, v.julia_cx, v.julia_cy,
+                gen_n=v.gen_n, gen_kr=v.gen_kr, gen_ki=v.gen_ki, palette_offset=v.palette_offset,
+                color_density=v.color_density, color_contrast=v.color_contrast,
+                d_cdf_lut=cdf if hist else None, num_bins=HISTOGRAM_BINS if hist else 2048,
+                is_hdr=bool(v.is_hdr), stream=dev.stream_handle, d_iter_map=frame.iter_map,
+                stream_slot=slot, d_counts=frame.counts, d_smooth=frame.smooth,
+                device=dev.index)
 
 
 def render_frame(params):

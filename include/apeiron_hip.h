@@ -201,6 +201,36 @@ int ap_render_fexp64(ap_ctx *ctx, const ap_view *view, const ap_orbit *orbit,
 int ap_render_fexp32(ap_ctx *ctx, const ap_view *view, const ap_orbit *orbit,
                      const void *bla, const ap_outputs *out);
 
+/* ---- Strict-Metric exhaustive fallback ------------------------------------
+ * Replaces eval_glitched_pixels_cuda_kernel / eval_glitched_pixels_hist_cuda_
+ * kernel (math_core/cuda_kernels.py:2470-2569) at precision tier 2, over
+ * compute_ml_sample (:2367-2467) and its 34-limb arithmetic (:2237-2364)
+ * (semantics AP_LIMBS_GPU34, limbs = 34), and the reference's CPU tier
+ * eval_glitched_pixels_fallback (math_core/engine.py:191-267) over
+ * compute_highprec_escape_scalar (math_core/high_precision.py:581-656), used
+ * there for widths below 1e-315 and for type 4 (semantics AP_LIMBS_HIGHPREC,
+ * limbs = 1 + ceil(prec_bits / 32)).  Tiers 0 and 1 (FP64, DD) are
+ * ap_render_fp64 / ap_render_dd over a queue.
+ * Numbers are signed magnitudes of `limbs` base-2^32 limbs, limb 0 the
+ * integer part, most significant first.  For each of the n listed pixels
+ * (queue[i] = y*W + x) coords holds the limbs of |c_re| then |c_im|
+ * (n x 2 x limbs uint32) and signs their signs (n x 2 int32, 1 = negative);
+ * params holds julia_cx, julia_cy, gen_kr, gen_ki (4 x limbs) and
+ * param_signs their signs (4).  All arrays are device memory.  Writes
+ * counts, smooth, iter_map, glitch (0) and rgb (palette scheme or the
+ * histogram cdf) of the listed pixels only: semantics 0 stores -1 in
+ * iter_map for the interior and colours the float32 value; semantics 1
+ * stores max_iter and draws the interior black, colouring the float64
+ * value, as the reference's CPU tier does.  Pixel coordinates, Julia
+ * parameters and k must be below 2^10 in magnitude. */
+#define AP_LIMBS_GPU34 0
+#define AP_LIMBS_HIGHPREC 1
+#define AP_MAX_LIMBS 256
+int ap_render_limbs(ap_ctx *ctx, const ap_view *view, int32_t semantics, int32_t limbs,
+                    const int32_t *queue, int32_t n, const uint32_t *coords,
+                    const int32_t *signs, const uint32_t *params,
+                    const int32_t *param_signs, const ap_outputs *out);
+
 /* ---- batched frames ------------------------------------------------------
  * No reference counterpart: the reference renders one frame per launch
  * (math_core/engine.py:1742-1806 for FloatExp) and a video one frame at a

@@ -787,28 +787,36 @@ double pymod(double a, double b) {
 }
 double tri(double ph) { return 1.0 - fabs((ph < 0.0 ? ph + 2.0 : ph) - 1.0); }
 
-// cpu_palette_norm, math_core/cpu_kernels.py:14-78 (sqrt of the float32
-// sample in float32).
-double palette_norm(float s, int scheme, double off, double density, double contrast) {
+// Square root of the sample in its own precision: Numba compiles
+// cpu_palette_norm once per argument type, so math.sqrt of a float32 sample
+// is a float32 square root and of a float64 sample a float64 one.
+double root(float s) { return (double)sqrtf(s); }
+double root(double s) { return sqrt(s); }
+
+// cpu_palette_norm, math_core/cpu_kernels.py:14-78, for a float32 sample (the
+// GPU passes) or a float64 one (the arbitrary-precision fallback,
+// math_core/engine.py:256-264).
+template <class S>
+double palette_norm(S s, int scheme, double off, double density, double contrast) {
     double dens = density <= 0.0 ? 1.0 : density;
     double cont = contrast <= 0.0 ? 1.0 : contrast;
     double sd = s;
     switch (scheme) {
     case 1: return tri(pymod(log(sd * cont + 1.0) * (0.45 * dens) + off * 2.0, 2.0));
     case 2: {
-        double t = tri(pymod((double)sqrtf(s) * (0.125 * dens) + off * 2.0, 2.0));
+        double t = tri(pymod(root(s) * (0.125 * dens) + off * 2.0, 2.0));
         double k = 4.0 * cont, td = tanh(0.5 * k);
         double sc = fabs(td) > 1e-7 ? 0.5 / td : 0.5186574;
         return 0.5 + tanh((t - 0.5) * k) * sc;
     }
     case 3: {
-        double t = tri(pymod((double)sqrtf(s) * (0.125 * dens) + off * 2.0, 2.0));
+        double t = tri(pymod(root(s) * (0.125 * dens) + off * 2.0, 2.0));
         double k = 3.0 * cont, ed = exp(k) - 1.0;
         double sc = fabs(ed) > 1e-7 ? 1.0 / ed : 0.0523957;
         return (exp(k * t) - 1.0) * sc;
     }
     case 4: {
-        double t = tri(pymod((double)sqrtf(s) * (0.5 * dens) + off * 2.0, 2.0));
+        double t = tri(pymod(root(s) * (0.5 * dens) + off * 2.0, 2.0));
         return cont != 1.0 ? pow(t, cont) : t;
     }
     case 5: {
@@ -829,7 +837,7 @@ double palette_norm(float s, int scheme, double off, double density, double cont
         return 0.5 + (ns > 1e-7 ? 0.5 / ns : 0.26666666666666666) * w;
     }
     case 8: {
-        double t = tri(pymod((double)sqrtf(s) * (0.1 * dens) + off * 2.0, 2.0));
+        double t = tri(pymod(root(s) * (0.1 * dens) + off * 2.0, 2.0));
         if (cont != 1.0) t = pow(t, cont);
         double t2 = t * t;
         return t2 * t * (t * (t * 6.0 - 15.0) + 10.0);
@@ -843,14 +851,15 @@ double palette_norm(float s, int scheme, double off, double density, double cont
         return raw >= 0.0 ? raw : raw + 1.0;
     }
     default: {
-        double t = tri(pymod((double)sqrtf(s) * (0.125 * dens) + off * 2.0, 2.0));
+        double t = tri(pymod(root(s) * (0.125 * dens) + off * 2.0, 2.0));
         return cont != 1.0 ? pow(t, cont) : t;
     }
     }
 }
 
 // cpu_sample_cdf_norm, math_core/cpu_kernels.py:1684-1706.
-double cdf_norm(float s, int max_iter, const float* cdf, int nb, double off, double density,
+template <class S>
+double cdf_norm(S s, int max_iter, const float* cdf, int nb, double off, double density,
                 double contrast) {
     double inv_max = 1.0 / std::max(1.0, (double)max_iter);
     double pos = ((double)s * inv_max) * (double)(nb - 1);
@@ -881,7 +890,8 @@ void lut_lerp(const float* lut, double norm, double& r, double& g, double& b) {
     b = (double)lut[i0 * 3 + 2] * inv + (double)lut[i1 * 3 + 2] * frac;
 }
 
-void colour(const ap_view& v, const float* cdf, int nb, const float* lut, float s, double& r,
+template <class S>
+void colour(const ap_view& v, const float* cdf, int nb, const float* lut, S s, double& r,
             double& g, double& b) {
     double norm = cdf ? cdf_norm(s, v.max_iter, cdf, nb, v.palette_offset, v.color_density,
                                  v.color_contrast)
@@ -902,6 +912,313 @@ void put_rgb(void* rgb, int hdr, size_t idx, double r, double g, double b) {
         p[1] = (uint8_t)(int)(g + 0.5);
         p[2] = (uint8_t)(int)(b + 0.5);
     }
+}
+
+// ------------------------------------------------ multi-limb fixed point ----
+// Signed-magnitude numbers of L base-2^32 limbs: limb 0 is the integer part,
+// limbs 1..L-1 the fraction, most significant first
+// (math_core/cuda_kernels.py:2237-2364).  The reference's GPU tier fixes
+// L = 34; the same arithmetic at larger L is this build's replacement for
+// the reference's arbitrary-precision CPU tier.
+const int kMaxLimbs = AP_MAX_LIMBS;
+
+// ml_cmp_mag, math_core/cuda_kernels.py:2289-2296.
+int ml_cmp(const uint32_t* a, const uint32_t* b, int L) {
+    for (int i = 0; i < L; ++i) {
+        if (a[i] > b[i]) return 1;
+        if (a[i] < b[i]) return -1;
+    }
+    return 0;
+}
+
+// ml_add_mag, :2299-2306; the carry out of limb 0 is returned and dropped by
+// every caller.
+uint32_t ml_add(const uint32_t* a, const uint32_t* b, uint32_t* out, int L) {
+    uint64_t carry = 0;
+    for (int i = L - 1; i >= 0; --i) {
+        uint64_t s = (uint64_t)a[i] + (uint64_t)b[i] + carry;
+        out[i] = (uint32_t)s;
+        carry = s >> 32;
+    }
+    return (uint32_t)carry;
+}
+
+// ml_sub_mag, :2309-2320 (a >= b).
+void ml_sub(const uint32_t* a, const uint32_t* b, uint32_t* out, int L) {
+    int64_t borrow = 0;
+    for (int i = L - 1; i >= 0; --i) {
+        int64_t d = (int64_t)a[i] - (int64_t)b[i] - borrow;
+        if (d < 0) {
+            d += 4294967296LL;
+            borrow = 1;
+        } else {
+            borrow = 0;
+        }
+        out[i] = (uint32_t)d;
+    }
+}
+
+// ml_add_signed, :2323-2334; returns the sign of the result (1 = negative).
+int ml_add_signed(int as, const uint32_t* a, int bs, const uint32_t* b, uint32_t* out, int L) {
+    if (as == bs) {
+        ml_add(a, b, out, L);
+        return as;
+    }
+    if (ml_cmp(a, b, L) >= 0) {
+        ml_sub(a, b, out, L);
+        return as;
+    }
+    ml_sub(b, a, out, L);
+    return bs;
+}
+
+// ml_sub_signed, :2337-2340.
+int ml_sub_signed(int as, const uint32_t* a, int bs, const uint32_t* b, uint32_t* out, int L) {
+    return ml_add_signed(as, a, bs == 0 ? 1 : 0, b, out, L);
+}
+
+// ml_mul_mag, :2343-2364: the product truncated to L limbs.  Column k sums
+// the low words of a_i*b_j over i+j = k and the high words over i+j = k+1;
+// the last column adds the high words of the first dropped column (i+j = L),
+// whose low words and their carries are dropped.  out must not alias a or b.
+void ml_mul(const uint32_t* a, const uint32_t* b, uint32_t* out, int L) {
+    uint64_t carry = 0;
+    for (int k = L - 1; k >= 0; --k) {
+        uint64_t sum = carry;
+        for (int i = 0; i <= k; ++i) sum += (uint64_t)(uint32_t)((uint64_t)a[i] * b[k - i]);
+        const int kp = k + 1;
+        if (kp < L) {
+            for (int i = 0; i <= kp; ++i) sum += ((uint64_t)a[i] * b[kp - i]) >> 32;
+        } else {
+            for (int i = 1; i < L; ++i) sum += ((uint64_t)a[i] * b[L - i]) >> 32;
+        }
+        out[k] = (uint32_t)sum;
+        carry = sum >> 32;
+    }
+}
+
+struct Mp {
+    int sign;
+    uint32_t l[kMaxLimbs];
+};
+
+void mp_load(Mp& x, int sign, const uint32_t* limbs, int L) {
+    x.sign = sign;
+    memcpy(x.l, limbs, sizeof(uint32_t) * L);
+}
+
+// Value of a magnitude as a double: the top 64 bits rounded once, plus the
+// third limb.
+double mp_double(const uint32_t* l, int L) {
+    uint64_t top = ((uint64_t)l[0] << 32) | (L > 1 ? l[1] : 0u);
+    double d = (double)top * 2.3283064365386963e-10;
+    if (L > 2) d += (double)l[2] * 5.421010862427522e-20;
+    return d;
+}
+double mp_signed_double(const Mp& x, int L) {
+    double d = mp_double(x.l, L);
+    return x.sign ? -d : d;
+}
+
+// Signed product: magnitude by ml_mul, sign the xor of the signs.
+void mp_mul(const Mp& a, const Mp& b, Mp& out, int L) {
+    ml_mul(a.l, b.l, out.l, L);
+    out.sign = a.sign != b.sign ? 1 : 0;
+}
+void mp_add(const Mp& a, const Mp& b, Mp& out, int L) {
+    out.sign = ml_add_signed(a.sign, a.l, b.sign, b.l, out.l, L);
+}
+void mp_sub(const Mp& a, const Mp& b, Mp& out, int L) {
+    out.sign = ml_sub_signed(a.sign, a.l, b.sign, b.l, out.l, L);
+}
+bool mp_is_zero(const Mp& x, int L) {
+    for (int i = 0; i < L; ++i)
+        if (x.l[i]) return false;
+    return true;
+}
+// Equality of values (-0 == +0), as MPFR compares.
+bool mp_equal(const Mp& a, const Mp& b, int L) {
+    if (ml_cmp(a.l, b.l, L) != 0) return false;
+    return a.sign == b.sign || mp_is_zero(a, L);
+}
+
+// One pixel of the exhaustive fallback.  s is the float32 map value, smooth
+// the float64 one (-1 interior), surv the value the arbitrary-precision tier
+// colours from (float64; >= max_iter or < 0 is drawn black).
+struct LimbRes {
+    int count;
+    float s;
+    double smooth;
+    double surv;
+};
+
+LimbRes limb_escape(const ap_view& v, int k, double mag2, bool gpu34) {
+    LimbRes r;
+    r.count = k;
+    double ir, ie;
+    inv_ln(v, ir, ie);
+    double lm = log(mag2);
+    if (lm > 1.0) {
+        double ll = log(0.5 * lm);
+        r.smooth = (double)(k + 1) - (ll - 1.7129286210981716) * ie;
+        if (gpu34) {
+            r.s = (float)(k + 1.0) - (float)((ll - v.ln_bailout_log) * v.inv_ln2);
+            r.surv = r.s;
+        } else {
+            // float(k + 1.0) - (log(0.5 ln mag2) - LN_BAILOUT_LOG) * inv_ln_n in
+            // float64, inv_ln_n = 1/ln n for type 4 (math_core/high_precision.py:
+            // 614, 645), rounded to float32 only when stored in the map.
+            double inv = (v.fractal_type == 4 && v.gen_n > 1) ? 1.0 / log((double)v.gen_n) : v.inv_ln2;
+            r.surv = (k + 1.0) - (ll - v.ln_bailout_log) * inv;
+            r.s = (float)r.surv;
+        }
+    } else {
+        r.smooth = (double)k;
+        r.surv = (double)k;
+        r.s = (float)k;
+    }
+    return r;
+}
+
+LimbRes limb_interior(const ap_view& v, bool gpu34) {
+    LimbRes r;
+    r.count = v.max_iter;
+    r.smooth = -1.0;
+    // The GPU tier returns -1; the arbitrary-precision tier returns
+    // float(max_iter) and stores it in the map (math_core/engine.py:250-251).
+    r.surv = gpu34 ? -1.0 : (double)v.max_iter;
+    r.s = (float)r.surv;
+    return r;
+}
+
+// compute_ml_sample, math_core/cuda_kernels.py:2367-2467 (gpu34 = true): the
+// escape test on the integer limbs, the smooth value from the top two limbs
+// of |z|^2; type 4 is not evaluated (-1).
+// compute_highprec_escape_scalar, math_core/high_precision.py:581-656
+// (gpu34 = false), in L-limb fixed point instead of MPFR: escape when
+// |z|^2 > 65536 exactly, the period check of types 0/2 on exact equality of
+// z, and for the magnitudes fixed point cannot hold (|z| >= 2^15 before a
+// square, |z^m| >= 2^22 inside the power of type 4) the escape is certain and
+// |z|^2 is taken in float64 from the leading limbs.
+LimbRes limb_sample(const ap_view& v, bool gpu34, int L, const Mp& px, const Mp& py,
+                    const Mp& jx, const Mp& jy, const Mp& kr, const Mp& ki) {
+    const int type = v.fractal_type;
+    if (gpu34 && type == 4) {
+        LimbRes r;
+        r.count = -1;
+        r.s = -1.0f;
+        r.smooth = -1.0;
+        r.surv = -1.0;
+        return r;
+    }
+    static thread_local Mp zx, zy, cx, cy, zx2, zy2, zxy, two, sub, t0, t1, t2, t3, zr, zi, sx, sy,
+        kzr, kzi, m;
+    if (type == 2 || type == 3) {
+        zx = px; zy = py; cx = jx; cy = jy;
+    } else {
+        memset(zx.l, 0, sizeof(uint32_t) * L);
+        memset(zy.l, 0, sizeof(uint32_t) * L);
+        zx.sign = zy.sign = 0;
+        cx = px; cy = py;
+    }
+    const bool period = !gpu34 && (type == 0 || type == 2);
+    int check_step = 8, saved_k = 0;
+    sx = zx;
+    sy = zy;
+    for (int k = 0; k < v.max_iter; ++k) {
+        if (gpu34 && (type == 1 || type == 3)) zx.sign = zy.sign = 0;
+        if (!gpu34 && (zx.l[0] >= (1u << 15) || zy.l[0] >= (1u << 15))) {
+            double dx = mp_double(zx.l, L), dy = mp_double(zy.l, L);
+            return limb_escape(v, k, dx * dx + dy * dy, false);
+        }
+        ml_mul(zx.l, zx.l, zx2.l, L);
+        ml_mul(zy.l, zy.l, zy2.l, L);
+        zx2.sign = zy2.sign = 0;
+        if (gpu34) {
+            const uint32_t x0 = zx2.l[0], y0 = zy2.l[0];
+            if (x0 >= 65536u || y0 >= 65536u || (uint64_t)x0 + y0 >= 65536u) {
+                double mag2 = (double)x0 + (double)y0 +
+                              ((double)zx2.l[1] + (double)zy2.l[1]) * 2.3283064365386963e-10;
+                return limb_escape(v, k, mag2, true);
+            }
+        } else {
+            ml_add(zx2.l, zy2.l, m.l, L);
+            bool over = m.l[0] > 65536u;
+            if (m.l[0] == 65536u)
+                for (int i = 1; i < L && !over; ++i) over = m.l[i] != 0;
+            if (over) return limb_escape(v, k, mp_double(m.l, L), false);
+            if (period) {
+                if (k == check_step) {
+                    check_step <<= 1;
+                    saved_k = k;
+                    sx = zx;
+                    sy = zy;
+                } else if (k > saved_k && mp_equal(zx, sx, L) && mp_equal(zy, sy, L)) {
+                    return limb_interior(v, false);
+                }
+            }
+        }
+        if (type == 0 || type == 2) {
+            // zy = 2 zx zy + cy; zx = (zx^2 - zy^2) + cx
+            mp_mul(zx, zy, zxy, L);
+            ml_add(zxy.l, zxy.l, two.l, L);
+            two.sign = zxy.sign;
+            mp_sub(zx2, zy2, sub, L);
+            mp_add(sub, cx, zx, L);
+            mp_add(two, cy, zy, L);
+        } else if (type == 1 || type == 3) {
+            // zy = 2 |zx| |zy| - cy; zx = (zx^2 - zy^2) + cx
+            ml_mul(zx.l, zy.l, zxy.l, L);
+            ml_add(zxy.l, zxy.l, two.l, L);
+            two.sign = 0;
+            mp_sub(zx2, zy2, sub, L);
+            mp_add(sub, cx, zx, L);
+            mp_sub(two, cy, zy, L);
+        } else {
+            // z^n by repeated multiplication, + k z + c
+            zr = zx;
+            zi = zy;
+            bool bail = false;
+            for (int e = 0; e < v.gen_n - 1; ++e) {
+                if (zr.l[0] >= (1u << 22) || zi.l[0] >= (1u << 22)) {
+                    bail = true;
+                    break;
+                }
+                mp_mul(zr, zx, t0, L);
+                mp_mul(zi, zy, t1, L);
+                mp_mul(zr, zy, t2, L);
+                mp_mul(zi, zx, t3, L);
+                mp_sub(t0, t1, zr, L);
+                mp_add(t2, t3, zi, L);
+            }
+            if (bail) {
+                // |z^n| >= 2^22 - |k z| - |c| > 256: escaped at k + 1.
+                const double dx = mp_signed_double(zx, L), dy = mp_signed_double(zy, L);
+                double pr = dx, pi = dy;
+                for (int e = 0; e < v.gen_n - 1; ++e) {
+                    double t = pr * dx - pi * dy;
+                    pi = pr * dy + pi * dx;
+                    pr = t;
+                }
+                const double dkr = mp_signed_double(kr, L), dki = mp_signed_double(ki, L);
+                const double nx = pr + (dkr * dx - dki * dy) + mp_signed_double(cx, L);
+                const double ny = pi + (dkr * dy + dki * dx) + mp_signed_double(cy, L);
+                if (k + 1 < v.max_iter) return limb_escape(v, k + 1, nx * nx + ny * ny, false);
+                return limb_interior(v, false);
+            }
+            mp_mul(kr, zx, t0, L);
+            mp_mul(ki, zy, t1, L);
+            mp_sub(t0, t1, kzr, L);
+            mp_mul(kr, zy, t2, L);
+            mp_mul(ki, zx, t3, L);
+            mp_add(t2, t3, kzi, L);
+            mp_add(zr, kzr, t0, L);
+            mp_add(t0, cx, zx, L);
+            mp_add(zi, kzi, t1, L);
+            mp_add(t1, cy, zy, L);
+        }
+    }
+    return limb_interior(v, gpu34);
 }
 
 // ------------------------------------------------ pixel driver --------------
@@ -1208,6 +1525,71 @@ int or_colorize(const ap_view* v, const float* it, const float* cdf, int nb, con
         put_rgb(rgb, v->is_hdr, i, r, g, b);
     }
     return 0;
+}
+
+
+// The exhaustive fallback over a pixel list (eval_glitched_pixels_cuda_kernel
+// and eval_glitched_pixels_hist_cuda_kernel, math_core/cuda_kernels.py:
+// 2470-2569, for semantics 0; eval_glitched_pixels_fallback,
+// math_core/engine.py:191-267, for semantics 1).  coords holds for each pixel
+// the limbs of |c_re| then |c_im| (n x 2 x L), signs their signs; params the
+// limbs of julia_cx, julia_cy, gen_kr, gen_ki (4 x L) and psigns theirs.
+// Writes counts, smooth, iter_map, glitch (0) and RGB of the listed pixels.
+int or_render_limbs(const ap_view* v, int semantics, int L, const int32_t* queue, int n,
+                    const uint32_t* coords, const int32_t* signs, const uint32_t* params,
+                    const int32_t* psigns, const ap_outputs* o, int threads) {
+    if (L < 2 || L > kMaxLimbs) return -1;
+    const ap_view V = *v;
+    const bool gpu34 = semantics == 0;
+    static thread_local Mp px, py, jx, jy, kr, ki;
+    mp_load(jx, psigns[0], params, L);
+    mp_load(jy, psigns[1], params + L, L);
+    mp_load(kr, psigns[2], params + 2 * L, L);
+    mp_load(ki, psigns[3], params + 3 * L, L);
+    const Mp JX = jx, JY = jy, KR = kr, KI = ki;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int q = 0; q < n; ++q) {
+        Mp* x = new Mp;
+        Mp* y = new Mp;
+        mp_load(*x, signs[2 * q], coords + (size_t)(2 * q) * L, L);
+        mp_load(*y, signs[2 * q + 1], coords + (size_t)(2 * q + 1) * L, L);
+        LimbRes r = limb_sample(V, gpu34, L, *x, *y, JX, JY, KR, KI);
+        delete x;
+        delete y;
+        const size_t idx = (size_t)queue[q];
+        if (o->counts) o->counts[idx] = r.count;
+        if (o->smooth) o->smooth[idx] = r.smooth;
+        if (o->iter_map) o->iter_map[idx] = r.s;
+        if (o->glitch) o->glitch[idx] = 0;
+        if (o->rgb) {
+            double cr = 0, cg = 0, cb = 0;
+            if (gpu34) {
+                if (r.s >= 0.0f) colour(V, o->cdf, o->num_bins, o->lut, r.s, cr, cg, cb);
+            } else if (!(r.surv >= (double)V.max_iter || r.surv < 0.0)) {
+                colour(V, o->cdf, o->num_bins, o->lut, r.surv, cr, cg, cb);
+            }
+            put_rgb(o->rgb, V.is_hdr, idx, cr, cg, cb);
+        }
+    }
+    return 0;
+}
+
+// Limb arithmetic entry points for the tests (op: 0 cmp, 1 add, 2 sub,
+// 3 mul, 4 add_signed, 5 sub_signed); returns the cmp result or the sign.
+int or_limb_op(int op, int L, int as, const uint32_t* a, int bs, const uint32_t* b,
+               uint32_t* out) {
+    switch (op) {
+    case 0: return ml_cmp(a, b, L);
+    case 1: return (int)ml_add(a, b, out, L);
+    case 2: ml_sub(a, b, out, L); return 0;
+    case 3: ml_mul(a, b, out, L); return 0;
+    case 4: return ml_add_signed(as, a, bs, b, out, L);
+    case 5: return ml_sub_signed(as, a, bs, b, out, L);
+    }
+    return -1;
 }
 
 }  // extern "C"

@@ -45,6 +45,8 @@ def load(fast=False):
         "or_edge_mask": (I, [P, I, I, C.c_float, P]),
         "or_hist_build": (I, [P, C.c_longlong, I, I, P]),
         "or_colorize": (I, [V, P, P, I, P, P]),
+        "or_render_limbs": (I, [V, I, I, P, I, P, P, P, P, O, I]),
+        "or_limb_op": (I, [I, I, I, P, I, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -158,3 +160,34 @@ def colorize(view, iter_map, lut, cdf=None):
     rgb = np.zeros(it.shape + (3,), np.uint16 if view.is_hdr else np.uint8)
     load().or_colorize(C.byref(view), _p(it), _p(cdf), 0 if cdf is None else len(cdf), _p(lut), _p(rgb))
     return rgb
+
+
+def render_limbs(view, semantics, limbs, queue, coords, signs, params, psigns, lut=None, cdf=None,
+                 maps=None, threads=0):
+    """Exhaustive-fallback evaluation of the listed pixels (flat indices
+    `queue`) in `limbs`-limb fixed point; semantics 0 = the reference's
+    34-limb GPU tier, 1 = its arbitrary-precision CPU tier.  Returns Maps
+    (only the listed pixels are written)."""
+    lib = load()
+    m = maps if maps is not None else Maps(view, rgb=lut is not None)
+    o = m.outputs(lut, None, cdf)
+    q = np.ascontiguousarray(queue, np.int32)
+    co = np.ascontiguousarray(coords, np.uint32)
+    sg = np.ascontiguousarray(signs, np.int32)
+    pa = np.ascontiguousarray(params, np.uint32)
+    ps = np.ascontiguousarray(psigns, np.int32)
+    r = lib.or_render_limbs(C.byref(view), semantics, limbs, _p(q), len(q), _p(co), _p(sg), _p(pa),
+                            _p(ps), C.byref(o), threads)
+    if r != 0:
+        raise ValueError(f"or_render_limbs failed ({r})")
+    return m
+
+
+def limb_op(op, a, b, a_sign=0, b_sign=0):
+    """One limb operation (0 cmp, 1 add, 2 sub, 3 mul, 4 add_signed,
+    5 sub_signed) on uint32 limb vectors; returns (result int, out limbs)."""
+    a = np.ascontiguousarray(a, np.uint32)
+    b = np.ascontiguousarray(b, np.uint32)
+    out = np.zeros_like(a)
+    r = load().or_limb_op(op, len(a), a_sign, _p(a), b_sign, _p(b), _p(out))
+    return r, out

@@ -56,7 +56,14 @@ def views():
         dict(name="config4", cx=c4["center_x"], cy=c4["center_y"], width=c4["width"],
              W=c4["size"][0], H=c4["size"][1], max_iter=c4["max_iter"], type=1, digits=120,
              samples=1024),
-    ] + [config5_frame(cfg["config5"], f) for f in (12, 31, 47, 63)]
+    ] + [config5_frame(cfg["config5"], f) for f in (12, 31, 47, 63)] + [
+        # Width 1e-250 (the 34-limb tier of the exhaustive fallback): the
+        # Misiurewicz point i, and the Burning Ship tip -2 (0 -> -2 -> 2 -> 2).
+        dict(name="tier2_misiurewicz_i_1e-250", cx="0", cy="1", width="1e-250", W=640, H=360,
+             max_iter=5000, type=0, digits=300, samples=512),
+        dict(name="tier2_bship_tip_1e-250", cx="-2", cy="0", width="1e-250", W=640, H=360,
+             max_iter=5000, type=1, digits=300, samples=512),
+    ]
 
 
 def config5_frame(c5, f):
