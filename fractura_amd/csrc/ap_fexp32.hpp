@@ -190,6 +190,16 @@ __device__ Sample fexp32_tail(const ap_view& v, const Fexp32Data& D, const PertC
             double ddy = (ry + dzy) * sys >= 0.0 ? sys * dzy : -sys * (2.0 * ry + dzy);
             dzy = 2.0 * (fabs(rx) * ddy + fabs(ry) * ddx + ddx * ddy) - dpy;
             dzx = ndx;
+        } else if (n == 3) {
+            // (Z+d)^3 - Z^3 + k d = d ((3 Z^2 + k) + d (3 Z + d)), in FMA form.
+            const double ux = fma(3.0, rx, dzx), uy = fma(3.0, ry, dzy);
+            const double ax = fma(3.0, fma(rx, rx, -(ry * ry)), v.gen_kr);
+            const double ay = fma(3.0, 2.0 * rx * ry, v.gen_ki);
+            const double wx = fma(dzx, ux, fma(-dzy, uy, ax));
+            const double wy = fma(dzx, uy, fma(dzy, ux, ay));
+            const double ndx = fma(dzx, wx, fma(-dzy, wy, dpx));
+            dzy = fma(dzx, wy, fma(dzy, wx, dpy));
+            dzx = ndx;
         } else {
             double pr = dzx, pi = dzy, qr = rx, qi = ry;
             for (int e = 0; e < n - 1; ++e) {
@@ -312,6 +322,18 @@ __device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si
             float ly = 2.0f * (fabsf(Z.x) * ddy + fabsf(Z.y) * ddx + ddx * ddy * s_nl) - cys;
             dzx = lx;
             dzy = ly;
+        } else if (n == 3) {
+            // ((Z+d)^3 - Z^3 + k d) / s = dz ((3 Z^2 + k) + d (3 Z + d)), d = dz s,
+            // s = 2^(z_exp-E), in FMA form.
+            const float tx = dzx * s_nl, ty = dzy * s_nl;
+            const float ux = fmaf(3.0f, Z.x, tx), uy = fmaf(3.0f, Z.y, ty);
+            const float ax = fmaf(3.0f, fmaf(Z.x, Z.x, -(Z.y * Z.y)), kr);
+            const float ay = fmaf(3.0f, 2.0f * Z.x * Z.y, ki);
+            const float wx = fmaf(tx, ux, fmaf(-ty, uy, ax));
+            const float wy = fmaf(tx, uy, fmaf(ty, ux, ay));
+            const float ndx = fmaf(dzx, wx, fmaf(-dzy, wy, cxs));
+            dzy = fmaf(dzx, wy, fmaf(dzy, wx, cys));
+            dzx = ndx;
         } else {
             float pr = dzx, pi = dzy, qr = Z.x, qi = Z.y;
             for (int e = 0; e < n - 1; ++e) {

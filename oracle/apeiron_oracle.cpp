@@ -632,6 +632,16 @@ Res fexp32_tail(const ap_view& v, const Orbit& o, double dzx, double dzy, double
             double ddy = (ry + dzy) * sys >= 0.0 ? sys * dzy : -sys * (2.0 * ry + dzy);
             dzy = 2.0 * (fabs(rx) * ddy + fabs(ry) * ddx + ddx * ddy) - dpy;
             dzx = ndx;
+        } else if (v.gen_n == 3) {
+            // (Z+d)^3 - Z^3 + k d = d ((3 Z^2 + k) + d (3 Z + d)), in FMA form.
+            const double ux = fma(3.0, rx, dzx), uy = fma(3.0, ry, dzy);
+            const double ax = fma(3.0, fma(rx, rx, -(ry * ry)), v.gen_kr);
+            const double ay = fma(3.0, 2.0 * rx * ry, v.gen_ki);
+            const double wx = fma(dzx, ux, fma(-dzy, uy, ax));
+            const double wy = fma(dzx, uy, fma(dzy, ux, ay));
+            const double ndx = fma(dzx, wx, fma(-dzy, wy, dpx));
+            dzy = fma(dzx, wy, fma(dzy, wx, dpy));
+            dzx = ndx;
         } else {
             double pr = dzx, pi = dzy, qr = rx, qi = ry;
             for (int e = 0; e < v.gen_n - 1; ++e) {
@@ -756,6 +766,17 @@ Res fexp32_sample(const ap_view& v, const Orbit32& O, double si, double sj) {
             float ly = 2.0f * (fabsf(Zx) * ddy + fabsf(Zy) * ddx + ddx * ddy * s_nl) - cys;
             dzx = lx;
             dzy = ly;
+        } else if (v.gen_n == 3) {
+            // ((Z+d)^3 - Z^3 + k d) / s = dz ((3 Z^2 + k) + d (3 Z + d)), d = dz s.
+            const float tx = dzx * s_nl, ty = dzy * s_nl;
+            const float ux = fmaf(3.0f, Zx, tx), uy = fmaf(3.0f, Zy, ty);
+            const float ax = fmaf(3.0f, fmaf(Zx, Zx, -(Zy * Zy)), kr);
+            const float ay = fmaf(3.0f, 2.0f * Zx * Zy, ki);
+            const float wx = fmaf(tx, ux, fmaf(-ty, uy, ax));
+            const float wy = fmaf(tx, uy, fmaf(ty, ux, ay));
+            const float ndx = fmaf(dzx, wx, fmaf(-dzy, wy, cxs));
+            dzy = fmaf(dzx, wy, fmaf(dzy, wx, cys));
+            dzx = ndx;
         } else {
             float pr = dzx, pi = dzy, qr = Zx, qi = Zy;
             for (int e = 0; e < v.gen_n - 1; ++e) {
