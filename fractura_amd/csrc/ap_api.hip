@@ -320,8 +320,11 @@ int ap_colorize_batch(ap_ctx* c, const ap_frame* frames, const ap_frame* frames_
     if (!frames || !frames_dev || !lut || nframes <= 0)
         return fail(c, AP_ERR_ARG, "colour batch needs host and device frame arrays and a LUT");
     long long next = 0;
+    int scheme = frames[0].view.scheme_id;  // -1 once two frames' rules differ
     for (int i = 0; i < nframes; ++i) {
         const ap_frame& f = frames[i];
+        const int rule = f.view.scheme_id <= 10 ? f.view.scheme_id : 0;
+        if (scheme >= 0 && rule != (scheme <= 10 ? scheme : 0)) scheme = -1;
         if (!f.out.iter_map || !f.out.rgb || f.view.width <= 0 || f.view.height <= 0)
             return fail(c, AP_ERR_ARG, "frame %d: colouring needs iter_map and rgb", i);
         if (((uintptr_t)f.out.iter_map & 15) || ((uintptr_t)f.out.rgb & 15))
@@ -334,7 +337,7 @@ int ap_colorize_batch(ap_ctx* c, const ap_frame* frames, const ap_frame* frames_
     if (next > INT32_MAX) return fail(c, AP_ERR_ARG, "batch exceeds 2^31 colouring blocks");
     int r = use_device(c);
     if (r) return r;
-    return check(c, ap::launch_colorize_batch(frames_dev, nframes, (int)next, lut, c->stream),
+    return check(c, ap::launch_colorize_batch(frames_dev, nframes, (int)next, scheme, lut, c->stream),
                  "colorize batch");
 }
 

@@ -14,8 +14,15 @@
 //                     reach, then each level in place; each level is written
 //                     once, in the requested forms (and level kLowTop also
 //                     as A, B, r^2 for the first merge launch).
+//   bla_high_kernel   levels kLowTop+1 .. levels-1 in LDS.  Above kLowTop a
+//                     merge pairs k with k + 2^(d-1), both of the same
+//                     residue k mod 2^kLowTop, so every residue class is an
+//                     independent tree: a block loads level kLowTop of one
+//                     class (a chunk of it, with the halo the higher merges
+//                     reach) and builds the remaining levels in place.
 //   bla_merge_kernel  one higher level per launch over all k, reading level
-//                     d-1 from the table.
+//                     d-1 from the table (tables of more levels than the
+//                     LDS tree holds, and the batched build).
 #include "ap_bla.hpp"
 #include "ap_launch.hpp"
 
@@ -30,11 +37,12 @@ struct BlaMut {
     int levels;
     int ref_len;
     int forms;     // AP_BLA_FORM_* bits
-    int state_lo;  // levels [state_lo, levels - 1) are read back by the merges
+    int state_lo;  // levels [state_lo, state_hi) are read back by a later launch
+    int state_hi;
 };
 
 __host__ __device__ inline BlaMut bla_mut(void* table, int ref_len, int levels, int forms,
-                                          int state_lo) {
+                                          int state_lo, int state_hi) {
     BlaLayout L = bla_layout(ref_len, levels);
     char* p = (char*)table;
     BlaMut b;
@@ -47,6 +55,7 @@ __host__ __device__ inline BlaMut bla_mut(void* table, int ref_len, int levels, 
     b.ref_len = ref_len;
     b.forms = forms;
     b.state_lo = state_lo;
+    b.state_hi = state_hi;
     return b;
 }
 
@@ -165,7 +174,7 @@ __device__ inline Coef32 make_coef32(double4 c, double r2, int next_hi) {
 __device__ inline void store_entry(const BlaMut& t, int k, int d, const BlaEntry& e) {
     size_t i = (size_t)d * t.ref_len + k;
     const bool fp64 = (t.forms & AP_BLA_FORM_FP64) != 0;
-    if (fp64 || (d >= t.state_lo && d < t.levels - 1)) {
+    if (fp64 || (d >= t.state_lo && d < t.state_hi)) {
         t.coef[i] = e.c;
         t.r2[i] = e.r2;
     }
@@ -174,10 +183,10 @@ __device__ inline void store_entry(const BlaMut& t, int k, int d, const BlaEntry
         t.c32[i] = make_coef32(e.c, e.r2, top_level(k + (1 << d), t.ref_len, t.levels));
 }
 
-constexpr int kLowOut = 256;  // k written per block
+constexpr int kLowOut = 512;  // k written per block
 constexpr int kLowTop = 8;    // levels 0..kLowTop built in LDS
 constexpr int kLowSpan = kLowOut + (1 << kLowTop) - 1;
-constexpr int kLowThreads = 256;
+constexpr int kLowThreads = 512;
 constexpr int kLowPer = (kLowSpan + kLowThreads - 1) / kLowThreads;
 
 // Levels 0..top (top <= kLowTop) for k in [k0, k0 + kLowOut), k0 = blk *
@@ -262,9 +271,80 @@ __global__ __launch_bounds__(kLowThreads) void bla_low_batch_kernel(const ap_bla
     __shared__ double4 sc[kLowSpan];
     __shared__ double sr[kLowSpan];
     const ap_bla_job& J = jobs[job_of(jobs, njobs, blockIdx.x, false)];
-    const BlaMut t = bla_mut(J.table, J.orbit.len, levels, forms, top);
+    const BlaMut t = bla_mut(J.table, J.orbit.len, levels, forms, top, levels - 1);
     bla_low_block(t, J.orbit.re, J.orbit.im, J.fractal_type, J.gen_n, J.gen_kr, J.gen_ki, J.tol,
                   top, blockIdx.x - J.low_begin, sc, sr);
+}
+
+constexpr int kHighRes = 4;                     // adjacent residue classes per block
+constexpr int kHighOut = 256;                   // class members written per block
+constexpr int kHighMaxHalo = 127;               // levels - 1 - kLowTop <= 7
+constexpr int kHighSpan = kHighOut + kHighMaxHalo;
+constexpr int kHighThreads = 1024;
+constexpr int kHighPer = (kHighRes * kHighSpan + kHighThreads - 1) / kHighThreads;
+
+// Block (g, c): residues r = kHighRes g + rr, rr < kHighRes, members m in
+// [m0, m0 + span) with k = r + m 2^top, m0 = c kHighOut.  Item q of a level
+// is residue rr = q mod kHighRes of member i = q / kHighRes, so a wave's
+// loads and stores cover kHighRes adjacent k per member (whole 128-byte runs
+// of FP32 entries).  Level d of member i merges level d-1 of members i and
+// i + 2^(d-1-top); a member whose partner lies past the loaded span is needed
+// by no output of the block (or lies past ref_len - 2^d, hence is zero), so
+// it is computed as zero.
+__global__ __launch_bounds__(kHighThreads) void bla_high_kernel(BlaMut t, int top) {
+    __shared__ double4 sc[kHighRes * kHighSpan];
+    __shared__ double sr[kHighRes * kHighSpan];
+    const int R = 1 << top;
+    const int groups = R / kHighRes;
+    const int r0 = (int)(blockIdx.x % (unsigned)groups) * kHighRes;
+    const int m0 = (int)(blockIdx.x / (unsigned)groups) * kHighOut;
+    const int ref_len = t.ref_len;
+    if (r0 >= ref_len) return;
+    const int M0 = (ref_len - 1 - r0) / R + 1;  // members of the block's largest class
+    if (m0 >= M0) return;
+    const int halo = (1 << (t.levels - 1 - top)) - 1;
+    const int span = min(kHighOut + halo, M0 - m0);
+    const int items = kHighRes * span;
+    for (int q = threadIdx.x; q < items; q += kHighThreads) {
+        const int r = r0 + (q % kHighRes), i = q / kHighRes;
+        const int k = r + (m0 + i) * R;
+        if (k < ref_len) {
+            const size_t g = (size_t)top * ref_len + (size_t)k;
+            sc[q] = t.coef[g];
+            sr[q] = t.r2[g];
+        }
+    }
+    for (int d = top + 1; d < t.levels; ++d) {
+        const int sm = (1 << (d - 1 - top)) * kHighRes;
+        BlaEntry v[kHighPer];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kHighPer; ++j) {
+            const int q = threadIdx.x + j * kHighThreads;
+            const int k = r0 + (q % kHighRes) + (m0 + q / kHighRes) * R;
+            v[j] = bla_zero();
+            if (q < items && q + sm < items && k < ref_len - (1 << d)) {
+                BlaEntry e1, e2;
+                e1.c = sc[q];
+                e1.r2 = sr[q];
+                e2.c = sc[q + sm];
+                e2.r2 = sr[q + sm];
+                v[j] = bla_merge(e1, e2);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kHighPer; ++j) {
+            const int q = threadIdx.x + j * kHighThreads;
+            if (q < items) {
+                const int i = q / kHighRes;
+                const int k = r0 + (q % kHighRes) + (m0 + i) * R;
+                sc[q] = v[j].c;
+                sr[q] = v[j].r2;
+                if (i < kHighOut && k < ref_len) store_entry(t, k, d, v[j]);
+            }
+        }
+    }
 }
 
 // Entry k of level d from level d-1: lanes k and k+1 read adjacent words.
@@ -298,7 +378,8 @@ __global__ __launch_bounds__(256) void bla_merge_batch_kernel(const ap_bla_job* 
     const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
     if (g >= total) return;
     const ap_bla_job& J = jobs[job_of(jobs, njobs, g, true)];
-    bla_merge_entry(bla_mut(J.table, J.orbit.len, levels, forms, top), d, (int)(g - J.entry_begin));
+    bla_merge_entry(bla_mut(J.table, J.orbit.len, levels, forms, top, levels - 1), d,
+                    (int)(g - J.entry_begin));
 }
 
 __global__ __launch_bounds__(256) void bla_export_kernel(BlaView t, double* __restrict__ out) {
@@ -320,13 +401,26 @@ hipError_t launch_bla_build(const ap_orbit& orb, int type, int n, double kr, dou
                             double tol, int levels, int forms, void* table, hipStream_t s) {
     const int ref_len = orb.len;
     const int top = levels - 1 < kLowTop ? levels - 1 : kLowTop;
-    BlaMut t = bla_mut(table, ref_len, levels, forms, top);
+    // The LDS tree takes the levels above top when their halo fits and only
+    // the FP32 form is written (its 32-byte entries fill whole sectors even
+    // at the class stride; the FP64 form's 8-byte fields would not); only
+    // level top is then read back from the table.
+    const bool tree = levels - 1 > top && levels - 1 - top <= 7 && forms == AP_BLA_FORM_FP32;
+    BlaMut t = bla_mut(table, ref_len, levels, forms, top, tree ? top + 1 : levels - 1);
     // ceil((ref_len + 1) / kLowOut) blocks: the last one also covers Z_ref_len of zf.
     hipLaunchKernelGGL(bla_low_kernel, dim3(ref_len / kLowOut + 1), dim3(kLowThreads), 0, s, t,
                        orb.re, orb.im, type, n, kr, ki, tol, top);
-    const int blocks = (ref_len + 255) / 256;
-    for (int d = top + 1; d < levels; ++d)
-        hipLaunchKernelGGL(bla_merge_kernel, dim3(blocks), dim3(256), 0, s, t, d);
+    if (tree) {
+        const int R = 1 << top;
+        const int members = (ref_len + R - 1) / R;  // of residue 0, the largest class
+        const unsigned chunks = (unsigned)((members + kHighOut - 1) / kHighOut);
+        hipLaunchKernelGGL(bla_high_kernel, dim3(chunks * (unsigned)(R / kHighRes)),
+                           dim3(kHighThreads), 0, s, t, top);
+    } else {
+        const int blocks = (ref_len + 255) / 256;
+        for (int d = top + 1; d < levels; ++d)
+            hipLaunchKernelGGL(bla_merge_kernel, dim3(blocks), dim3(256), 0, s, t, d);
+    }
     return hipGetLastError();
 }
 

@@ -301,10 +301,32 @@ constexpr int kColPx = 4 * kColGroups * kColThreads;
 // int(x + 0.5) truncates toward zero, as Python int() does.
 __device__ inline unsigned channel(double x) { return (unsigned)(int)(x + 0.5); }
 
+// S >= 0: the colour rule is fixed at compile time (palette scheme S, or the
+// histogram CDF for S = kSchemeCdf), so a kernel carries one transfer
+// function instead of all twelve; S < 0: the view's scheme at run time.
+constexpr int kSchemeCdf = 11;
+constexpr int kSchemeAny = -1;
+
+template <int S>
+__device__ inline void sample_rgb(const ap_view& v, const ap_outputs& o, float s, double& r,
+                                  double& g, double& b) {
+    if (S < 0) {
+        sample_colour(v, o, s, r, g, b);
+    } else {
+        const double norm = S == kSchemeCdf
+                                ? cdf_norm(s, v.max_iter, o.cdf, o.num_bins, v.palette_offset,
+                                           v.color_density, v.color_contrast)
+                                : palette_norm(s, S, v.palette_offset, v.color_density,
+                                               v.color_contrast);
+        lut_lerp(o.lut, norm, r, g, b);
+    }
+}
+
+template <int S>
 __device__ inline void colour_px(const ap_view& v, const ap_outputs& o, float s, unsigned& r,
                                  unsigned& g, unsigned& b) {
     double dr = 0.0, dg = 0.0, db = 0.0;
-    if (s >= 0.0f) sample_colour(v, o, s, dr, dg, db);
+    if (s >= 0.0f) sample_rgb<S>(v, o, s, dr, dg, db);
     r = channel(dr);
     g = channel(dg);
     b = channel(db);
@@ -312,13 +334,14 @@ __device__ inline void colour_px(const ap_view& v, const ap_outputs& o, float s,
 
 // RGB of the 4-pixel group `grp` of the block into its LDS slot: 3 packed
 // dwords for 8-bit channels, 6 for 16-bit.
+template <int S>
 __device__ inline void colour_group(const ap_view& v, const ap_outputs& o, float4 q, int grp,
                                     unsigned* st) {
     unsigned r0, g0, b0, r1, g1, b1, r2, g2, b2, r3, g3, b3;
-    colour_px(v, o, q.x, r0, g0, b0);
-    colour_px(v, o, q.y, r1, g1, b1);
-    colour_px(v, o, q.z, r2, g2, b2);
-    colour_px(v, o, q.w, r3, g3, b3);
+    colour_px<S>(v, o, q.x, r0, g0, b0);
+    colour_px<S>(v, o, q.y, r1, g1, b1);
+    colour_px<S>(v, o, q.z, r2, g2, b2);
+    colour_px<S>(v, o, q.w, r3, g3, b3);
     if (v.is_hdr) {
         unsigned* w = st + 6 * grp;
         w[0] = (r0 & 0xffffu) | (g0 << 16);
@@ -336,6 +359,7 @@ __device__ inline void colour_group(const ap_view& v, const ap_outputs& o, float
 }
 
 // Block blk of a map of n pixels: pixels [blk * kColPx, (blk + 1) * kColPx).
+template <int S>
 __device__ __attribute__((always_inline)) inline void colour_block(const ap_view& v, const float* __restrict__ it,
                                     const float* __restrict__ cdf, int num_bins,
                                     const float* __restrict__ lut, void* __restrict__ rgb,
@@ -349,7 +373,7 @@ __device__ __attribute__((always_inline)) inline void colour_block(const ap_view
         for (long long p = pb + threadIdx.x; p < n; p += kColThreads) {
             const float s = it[p];
             double r = 0.0, g = 0.0, b = 0.0;
-            if (s >= 0.0f) sample_colour(v, o, s, r, g, b);
+            if (s >= 0.0f) sample_rgb<S>(v, o, s, r, g, b);
             store_rgb(rgb, v.is_hdr, p, r, g, b);
         }
         return;
@@ -367,7 +391,7 @@ __device__ __attribute__((always_inline)) inline void colour_block(const ap_view
 #pragma unroll 1
     for (int j = 0; j < kColGroups; ++j) {
         const float4 q = j == 0 ? q0 : (j == 1 ? q1 : (j == 2 ? q2 : q3));
-        colour_group(v, o, q, threadIdx.x + j * kColThreads, st);
+        colour_group<S>(v, o, q, threadIdx.x + j * kColThreads, st);
     }
     __syncthreads();
     const int words = v.is_hdr ? kColPx * 6 / 16 : kColPx * 3 / 16;
@@ -375,17 +399,19 @@ __device__ __attribute__((always_inline)) inline void colour_block(const ap_view
     for (int i = threadIdx.x; i < words; i += kColThreads) dst[i] = stage[i];
 }
 
+template <int S>
 __global__ __launch_bounds__(kColThreads) void colorize_kernel(ap_view v, const float* __restrict__ it,
                                                                const float* __restrict__ cdf,
                                                                int num_bins,
                                                                const float* __restrict__ lut,
                                                                void* __restrict__ rgb, long long n) {
     __shared__ uint4 stage[kColPx * 6 / 16];
-    colour_block(v, it, cdf, num_bins, lut, rgb, n, blockIdx.x, stage);
+    colour_block<S>(v, it, cdf, num_bins, lut, rgb, n, blockIdx.x, stage);
 }
 
 // Every frame of a batch: block b colours block b - colour_begin of the last
 // frame whose colour_begin <= b (a block-uniform binary search).
+template <int S>
 __global__ __launch_bounds__(kColThreads) void colorize_batch_kernel(const ap_frame* __restrict__ frames,
                                                                      int nframes,
                                                                      const float* __restrict__ lut) {
@@ -399,17 +425,47 @@ __global__ __launch_bounds__(kColThreads) void colorize_batch_kernel(const ap_fr
             hi = mid - 1;
     }
     const ap_frame& F = frames[lo];
-    colour_block(F.view, F.out.iter_map, nullptr, 0, lut, F.out.rgb,
+    colour_block<S>(F.view, F.out.iter_map, nullptr, 0, lut, F.out.rgb,
                  (long long)F.view.width * F.view.height, (long long)blockIdx.x - F.colour_begin,
                  stage);
 }
 
 static_assert(kColPx == kColourPixels, "colouring block size");
 
-hipError_t launch_colorize_batch(const ap_frame* frames_dev, int nframes, int blocks,
+// The compile-time rule of a colour pass: the CDF when one is given, else
+// the palette scheme (schemes above 10 fall back to 0, as palette_norm's
+// default case does).
+static int colour_rule(int scheme, bool cdf) {
+    if (cdf) return kSchemeCdf;
+    return scheme >= 0 && scheme <= 10 ? scheme : 0;
+}
+
+// One launch of kernel template K at rule `rule` (kSchemeAny when it is not
+// one of the fixed rules).
+#define AP_COLOUR_LAUNCH(K, rule, grid, s, ...)                                       \
+    do {                                                                             \
+        switch (rule) {                                                              \
+        case 0: hipLaunchKernelGGL(K<0>, grid, dim3(kColThreads), 0, s, __VA_ARGS__); break;   \
+        case 1: hipLaunchKernelGGL(K<1>, grid, dim3(kColThreads), 0, s, __VA_ARGS__); break;   \
+        case 2: hipLaunchKernelGGL(K<2>, grid, dim3(kColThreads), 0, s, __VA_ARGS__); break;   \
+        case 3: hipLaunchKernelGGL(K<3>, grid, dim3(kColThreads), 0, s, __VA_ARGS__); break;   \
+        case 4: hipLaunchKernelGGL(K<4>, grid, dim3(kColThreads), 0, s, __VA_ARGS__); break;   \
+        case 5: hipLaunchKernelGGL(K<5>, grid, dim3(kColThreads), 0, s, __VA_ARGS__); break;   \
+        case 6: hipLaunchKernelGGL(K<6>, grid, dim3(kColThreads), 0, s, __VA_ARGS__); break;   \
+        case 7: hipLaunchKernelGGL(K<7>, grid, dim3(kColThreads), 0, s, __VA_ARGS__); break;   \
+        case 8: hipLaunchKernelGGL(K<8>, grid, dim3(kColThreads), 0, s, __VA_ARGS__); break;   \
+        case 9: hipLaunchKernelGGL(K<9>, grid, dim3(kColThreads), 0, s, __VA_ARGS__); break;   \
+        case 10: hipLaunchKernelGGL(K<10>, grid, dim3(kColThreads), 0, s, __VA_ARGS__); break; \
+        case 11: hipLaunchKernelGGL(K<11>, grid, dim3(kColThreads), 0, s, __VA_ARGS__); break; \
+        default: hipLaunchKernelGGL(K<kSchemeAny>, grid, dim3(kColThreads), 0, s, __VA_ARGS__); \
+        }                                                                            \
+    } while (0)
+
+hipError_t launch_colorize_batch(const ap_frame* frames_dev, int nframes, int blocks, int scheme,
                                  const float* lut, hipStream_t s) {
-    hipLaunchKernelGGL(colorize_batch_kernel, dim3(blocks), dim3(kColThreads), 0, s, frames_dev,
-                       nframes, lut);
+    // scheme < 0: the frames' schemes differ, each block reads its frame's.
+    const int rule = scheme < 0 ? kSchemeAny : colour_rule(scheme, false);
+    AP_COLOUR_LAUNCH(colorize_batch_kernel, rule, dim3(blocks), s, frames_dev, nframes, lut);
     return hipGetLastError();
 }
 
@@ -418,8 +474,8 @@ hipError_t launch_colorize(const ap_view& v, const float* it, const float* cdf, 
     long long n = (long long)v.width * v.height;
     if (n <= 0) return hipSuccess;
     int blocks = (int)((n + kColPx - 1) / kColPx);
-    hipLaunchKernelGGL(colorize_kernel, dim3(blocks), dim3(kColThreads), 0, s, v, it, cdf, num_bins,
-                       lut, rgb, n);
+    const int rule = colour_rule(v.scheme_id, cdf != nullptr);
+    AP_COLOUR_LAUNCH(colorize_kernel, rule, dim3(blocks), s, v, it, cdf, num_bins, lut, rgb, n);
     return hipGetLastError();
 }
 

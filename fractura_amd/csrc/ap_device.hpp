@@ -155,8 +155,13 @@ __device__ inline dd dd_abs(dd a) {
 
 // ---------------------------------------------------------------- colour --
 // Python's float modulo (result takes the divisor's sign).
+// Every divisor here is 2 or 1: for a power of two b, a - b trunc(a / b) is
+// exactly fmod(a, b) (a / b and the product are exact, and the difference is
+// representable at a's precision), with fmod's zero taking a's sign; this
+// replaces the library's general fmod sequence.
 __device__ inline double pymod(double a, double b) {
-    double m = fmod(a, b);
+    double m = a - b * trunc(a / b);
+    if (m == 0.0) m = copysign(0.0, a);
     if (m != 0.0 && ((b < 0.0) != (m < 0.0))) m += b;
     return m;
 }

@@ -52,7 +52,7 @@ hipError_t launch_hist_build(const float* iter_map, int n, int max_iter, int num
 hipError_t launch_hist_cdf(const int* hist, int num_bins, float* cdf, hipStream_t s);
 hipError_t launch_colorize(const ap_view& v, const float* iter_map, const float* cdf,
                            int num_bins, const float* lut, void* rgb, hipStream_t s);
-hipError_t launch_colorize_batch(const ap_frame* frames_dev, int nframes, int blocks,
+hipError_t launch_colorize_batch(const ap_frame* frames_dev, int nframes, int blocks, int scheme,
                                  const float* lut, hipStream_t s);
 constexpr int kColourPixels = 4096;  // pixels per colouring block
 
