@@ -172,34 +172,50 @@ __device__ inline unsigned long long wave_sum(unsigned long long x) {
 // column (i+j = L).  As an integer that is
 //   sum_{q<L} F_q 2^(32(L-1-q)) + H  (mod 2^(32L)),
 //   F_q = sum_{i+j=q} a_i b_j,   H = sum_{i+j=L} hi32(a_i b_j),
-// which is what is computed: F_q in 96 bits per lane, H by one reversed
-// product per lane and a wave sum, then two carry passes.
+// which is what is computed: F_q in 96 bits per lane (the column sums of the
+// loop over t below), H by one reversed product per lane and a wave sum, then
+// two carry passes.
 template <int P>
-__device__ inline void mul_mag(const uint32_t (&a)[P], const uint32_t (&b)[P], uint32_t (&out)[P],
-                               int L) {
-    const int ln = lane_id();
-    unsigned long long acc[P];
-    uint32_t top[P], bs[P];
+struct Cols {
+    unsigned long long acc[P];  // low 64 bits of F_q
+    uint32_t top[P];            // bits 64..95
+};
+
+template <int P>
+__device__ inline void cols_zero(Cols<P>& c) {
 #pragma unroll
     for (int m = 0; m < P; ++m) {
-        acc[m] = 0ull;
-        top[m] = 0u;
-        bs[m] = b[m];
+        c.acc[m] = 0ull;
+        c.top[m] = 0u;
     }
+}
+
+template <int P>
+__device__ inline void cols_add(Cols<P>& c, uint32_t at, const uint32_t (&bs)[P]) {
 #pragma unroll
-    for (int ma = 0; ma < P; ++ma) {
-        const int tend = min(64, L - 64 * ma);
-        for (int t = 0; t < tend; ++t) {
-            const uint32_t at = lane_value(a[ma], t);
-#pragma unroll
-            for (int m = 0; m < P; ++m) {
-                const unsigned long long s = acc[m] + (unsigned long long)at * bs[m];
-                top[m] += s < acc[m] ? 1u : 0u;
-                acc[m] = s;
-            }
-            shift_up(bs);
-        }
+    for (int m = 0; m < P; ++m) {
+        const unsigned long long s = c.acc[m] + (unsigned long long)at * bs[m];
+        c.top[m] += s < c.acc[m] ? 1u : 0u;
+        c.acc[m] = s;
     }
+}
+
+// c += d (96-bit column sums).
+template <int P>
+__device__ inline void cols_merge(Cols<P>& c, const Cols<P>& d) {
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const unsigned long long s = c.acc[m] + d.acc[m];
+        c.top[m] += d.top[m] + (s < c.acc[m] ? 1u : 0u);
+        c.acc[m] = s;
+    }
+}
+
+// The product a b from its column sums.
+template <int P>
+__device__ inline void cols_finish(const Cols<P>& c, const uint32_t (&a)[P], const uint32_t (&b)[P],
+                                   uint32_t (&out)[P], int L) {
+    const int ln = lane_id();
     // H: lane q (1 <= q <= L-1) multiplies b_q by a_(L-q).
     unsigned long long h = 0ull;
 #pragma unroll
@@ -222,9 +238,9 @@ __device__ inline void mul_mag(const uint32_t (&a)[P], const uint32_t (&b)[P], u
 #pragma unroll
     for (int m = 0; m < P; ++m) {
         const bool live = ln + 64 * m < L;
-        w0[m] = live ? (uint32_t)acc[m] : 0u;
-        w1[m] = live ? (uint32_t)(acc[m] >> 32) : 0u;
-        w2[m] = live ? top[m] : 0u;
+        w0[m] = live ? (uint32_t)c.acc[m] : 0u;
+        w1[m] = live ? (uint32_t)(c.acc[m] >> 32) : 0u;
+        w2[m] = live ? c.top[m] : 0u;
     }
     shift_down(w1);
     shift_down(w2);
@@ -246,6 +262,92 @@ __device__ inline void mul_mag(const uint32_t (&a)[P], const uint32_t (&b)[P], u
         g[m] = (u >> 32) != 0ull;
     }
     resolve<P, false>(out, g, L);
+}
+
+template <int P>
+__device__ inline void mul_mag(const uint32_t (&a)[P], const uint32_t (&b)[P], uint32_t (&out)[P],
+                               int L) {
+    Cols<P> c;
+    cols_zero(c);
+    uint32_t bs[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) bs[m] = b[m];
+#pragma unroll
+    for (int ma = 0; ma < P; ++ma) {
+        const int tend = min(64, L - 64 * ma);
+        for (int t = 0; t < tend; ++t) {
+            cols_add(c, lane_value(a[ma], t), bs);
+            shift_up(bs);
+        }
+    }
+    cols_finish(c, a, b, out, L);
+}
+
+// Products of two numbers by two others in one pass over t: p00 = a0 b0,
+// p11 = a1 b1, p01 = a0 b1 and, when kCross, p10 = a1 b0.  The three or four
+// multiply-accumulate chains are independent, so one wave overlaps them;
+// each result equals mul_mag's.
+template <int P, bool kCross>
+__device__ inline void mul22(const uint32_t (&a0)[P], const uint32_t (&a1)[P],
+                             const uint32_t (&b0)[P], const uint32_t (&b1)[P], uint32_t (&p00)[P],
+                             uint32_t (&p11)[P], uint32_t (&p01)[P], uint32_t (&p10)[P], int L) {
+    // Even and odd t accumulate into separate column sums: twice as many
+    // independent chains in flight, merged once at the end.
+    Cols<P> c00, c11, c01, c10, e00, e11, e01, e10;
+    cols_zero(c00);
+    cols_zero(c11);
+    cols_zero(c01);
+    cols_zero(e00);
+    cols_zero(e11);
+    cols_zero(e01);
+    if (kCross) {
+        cols_zero(c10);
+        cols_zero(e10);
+    }
+    uint32_t s0[P], s1[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        s0[m] = b0[m];
+        s1[m] = b1[m];
+    }
+#pragma unroll
+    for (int ma = 0; ma < P; ++ma) {
+        const int tend = min(64, L - 64 * ma);
+        int t = 0;
+        for (; t + 1 < tend; t += 2) {
+            const uint32_t x = lane_value(a0[ma], t), y = lane_value(a1[ma], t);
+            const uint32_t x1 = lane_value(a0[ma], t + 1), y1 = lane_value(a1[ma], t + 1);
+            cols_add(c00, x, s0);
+            cols_add(c11, y, s1);
+            cols_add(c01, x, s1);
+            if (kCross) cols_add(c10, y, s0);
+            shift_up(s0);
+            shift_up(s1);
+            cols_add(e00, x1, s0);
+            cols_add(e11, y1, s1);
+            cols_add(e01, x1, s1);
+            if (kCross) cols_add(e10, y1, s0);
+            shift_up(s0);
+            shift_up(s1);
+        }
+        if (t < tend) {
+            const uint32_t x = lane_value(a0[ma], t), y = lane_value(a1[ma], t);
+            cols_add(c00, x, s0);
+            cols_add(c11, y, s1);
+            cols_add(c01, x, s1);
+            if (kCross) cols_add(c10, y, s0);
+            shift_up(s0);
+            shift_up(s1);
+        }
+    }
+    cols_merge(c00, e00);
+    cols_merge(c11, e11);
+    cols_merge(c01, e01);
+    if (kCross) cols_merge(c10, e10);
+    cols_finish(c00, a0, b0, p00, L);
+    cols_finish(c11, a1, b1, p11, L);
+    cols_finish(c01, a0, b1, p01, L);
+    if (kCross) cols_finish(c10, a1, b0, p10, L);
 }
 
 template <int P>
@@ -428,8 +530,12 @@ __device__ LimbResult limb_sample(const ap_view& v, const LimbArgs& A, int px) {
                 return limb_escape(v, k, dx * dx + dy * dy, false);
             }
         }
-        mul_mag(zx.l, zx.l, zx2.l, L);
-        mul_mag(zy.l, zy.l, zy2.l, L);
+        // zx^2, zy^2 and |zx||zy| in one pass (the cross product is used by
+        // the update of types 0-3; its sign is set there).
+        {
+            uint32_t unused[P];
+            mul22<P, false>(zx.l, zy.l, zx.l, zy.l, zx2.l, zy2.l, zxy.l, unused, L);
+        }
         zx2.sign = 0;
         zy2.sign = 0;
         if (kGpu34) {
@@ -466,7 +572,7 @@ __device__ LimbResult limb_sample(const ap_view& v, const LimbArgs& A, int px) {
         }
         if (type == AP_MANDELBROT || type == AP_JULIA_MANDELBROT) {
             // zy = 2 zx zy + cy; zx = (zx^2 - zy^2) + cx
-            mul_s(zx, zy, zxy, L);
+            zxy.sign = zx.sign != zy.sign ? 1 : 0;
             Num<P> two;
             add_mag(zxy.l, zxy.l, two.l, L);
             two.sign = zxy.sign;
@@ -475,7 +581,6 @@ __device__ LimbResult limb_sample(const ap_view& v, const LimbArgs& A, int px) {
             add_s(two, cy, zy, L);
         } else if (type == AP_BURNING_SHIP || type == AP_JULIA_BURNING_SHIP) {
             // zy = 2 |zx| |zy| - cy; zx = (zx^2 - zy^2) + cx
-            mul_mag(zx.l, zy.l, zxy.l, L);
             Num<P> two;
             add_mag(zxy.l, zxy.l, two.l, L);
             two.sign = 0;
@@ -493,14 +598,15 @@ __device__ LimbResult limb_sample(const ap_view& v, const LimbArgs& A, int px) {
                     bail = true;
                     break;
                 }
-                mul_s(zr, zx, t0, L);
-                mul_s(zi, zy, t1, L);
-                Num<P> re;
-                sub_s(t0, t1, re, L);
-                mul_s(zr, zy, t0, L);
-                mul_s(zi, zx, t1, L);
-                add_s(t0, t1, zi, L);
-                copy(zr, re);
+                // zr zx, zi zy, zr zy, zi zx in one pass
+                Num<P> p2, p3;
+                mul22<P, true>(zr.l, zi.l, zx.l, zy.l, t0.l, t1.l, p2.l, p3.l, L);
+                t0.sign = zr.sign != zx.sign ? 1 : 0;
+                t1.sign = zi.sign != zy.sign ? 1 : 0;
+                p2.sign = zr.sign != zy.sign ? 1 : 0;
+                p3.sign = zi.sign != zx.sign ? 1 : 0;
+                sub_s(t0, t1, zr, L);
+                add_s(p2, p3, zi, L);
             }
             load(kr, A.params + 2 * L, A.psigns[2], L);
             load(ki, A.params + 3 * L, A.psigns[3], L);
@@ -519,15 +625,17 @@ __device__ LimbResult limb_sample(const ap_view& v, const LimbArgs& A, int px) {
                 if (k + 1 < v.max_iter) return limb_escape(v, k + 1, nx * nx + ny * ny, false);
                 return limb_interior(v, false);
             }
-            Num<P> kz;
-            mul_s(kr, zx, t0, L);
-            mul_s(ki, zy, t1, L);
+            // kr zx, ki zy, kr zy, ki zx in one pass
+            Num<P> kz, p2, p3;
+            mul22<P, true>(kr.l, ki.l, zx.l, zy.l, t0.l, t1.l, p2.l, p3.l, L);
+            t0.sign = kr.sign != zx.sign ? 1 : 0;
+            t1.sign = ki.sign != zy.sign ? 1 : 0;
+            p2.sign = kr.sign != zy.sign ? 1 : 0;
+            p3.sign = ki.sign != zx.sign ? 1 : 0;
             sub_s(t0, t1, kz, L);   // kz_r
             add_s(zr, kz, t0, L);   // zn_r + kz_r
-            mul_s(kr, zy, t1, L);
-            mul_s(ki, zx, kz, L);
-            add_s(t1, kz, zr, L);   // kz_i (zr is free)
-            add_s(zi, zr, t1, L);   // zn_i + kz_i
+            add_s(p2, p3, kz, L);   // kz_i
+            add_s(zi, kz, t1, L);   // zn_i + kz_i
             add_s(t0, cx, zx, L);
             add_s(t1, cy, zy, L);
         }
