@@ -1,0 +1,184 @@
+"""Edge cases of the pixel passes, the table build and the host entry points:
+frames that do not fill a 16x16 tile (1x1, 1xN, Nx1, ragged sizes),
+max_iter 0 and 1, empty pixel lists, orbits shorter than a BLA block, a frame
+too large for 32-bit pixel indices (rejected before any launch) and an
+8192x8192 frame (64-bit offsets).  Outputs are held to the bars of
+tests/test_gpu_direct.py and tests/test_gpu_pert.py; the FP32-only table
+build (levels above 8 as one LDS tree over residue classes) must equal the
+level-by-level build of the FP32 form byte for byte.
+"""
+import ctypes as C
+from decimal import Decimal
+
+import numpy as np
+import pytest
+
+from fractura_amd import _abi, engine, exhaustive
+from tests.helpers import lut, oracle_frame, to_np
+from tests.test_gpu_direct import _check_maps
+
+pytestmark = pytest.mark.gpu
+
+
+def _sync():
+    import torch
+
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("W,H", [(1, 1), (1, 37), (37, 1), (17, 5), (63, 65), (257, 3)])
+@pytest.mark.parametrize("eng", ["fp64", "dd"])
+def test_ragged_frames_match_oracle(W, H, eng):
+    plan = engine.plan_view(W, H, 300, Decimal("-0.65"), Decimal("0.1"), Decimal("3.0"),
+                            engine_override=eng, compute_orbit=False)
+    fr = engine.render_view(plan, "inferno", rebase=False)
+    _sync()
+    # RGB: within one level everywhere (libm vs device log); the share of
+    # exact pixels is only meaningful on frames of many pixels.
+    _check_maps(fr, oracle_frame(plan, lut()), rgb_exact_frac=0.99 if W * H >= 1000 else 0.0)
+
+
+@pytest.mark.parametrize("max_iter", [0, 1, 2])
+def test_tiny_max_iter(max_iter):
+    plan = engine.plan_view(40, 24, max_iter, Decimal("-0.65"), Decimal("0.0"), Decimal("4.0"),
+                            engine_override="fp64", compute_orbit=False)
+    fr = engine.render_view(plan, "inferno", rebase=False)
+    _sync()
+    m = oracle_frame(plan, lut())
+    _check_maps(fr, m)
+    counts = to_np(fr.counts)
+    assert np.all((counts == -1) | (counts <= max_iter))
+
+
+@pytest.mark.parametrize("W,H", [(1, 1), (19, 7), (130, 3)])
+@pytest.mark.parametrize("eng", ["pert64", "fexp32"])
+def test_ragged_perturbation_frames(W, H, eng):
+    from tests.test_gpu_pert import _oracle
+
+    kw = dict(bbsa_order=32, bbsa_tol=1e-6, glitch_mode="Pauldelbrot")
+    if eng == "fexp32":
+        kw.update(force_floatexp=True, fexp_mantissa=32)
+    plan = engine.plan_view(W, H, 3000, Decimal("0"), Decimal("1"), Decimal("1e-200"), **kw)
+    assert plan.engine == eng
+    fr = engine.render_view(plan, "inferno", rebase=False)
+    _sync()
+    if eng == "pert64":
+        m = _oracle(plan)
+        assert np.array_equal(to_np(fr.counts), m.counts)
+    else:
+        from oracle import oracle
+
+        re, im, n = plan.orbit
+        bla = oracle.bla_build(re, im, n, 0, plan.bla_tol, 16)
+        v = plan.view.copy()
+        v.mode = 0
+        m = oracle.render("fexp32", v, orbit=(re, im), bla=bla, levels=16)
+        assert np.mean(to_np(fr.counts) == m.counts) >= 0.99
+
+
+def _layout(ref_len, levels):
+    a = lambda b: (b + 255) & ~255
+    n = ref_len * levels
+    r2 = a(n * 32)
+    r1 = r2 + a(n * 8)
+    c32 = r1 + a(n * 8)
+    zf = c32 + a(n * 32)
+    return c32, zf, zf + a((ref_len + 1) * 8)
+
+
+@pytest.mark.parametrize("ref_len", [4, 5, 255, 256, 257, 511, 513, 1023, 4097, 70001, 200000])
+def test_fp32_table_tree_equals_level_by_level(ref_len):
+    """The FP32-only build takes the LDS tree for levels 9..15; the FP32 form
+    of an AP_BLA_FORM_ALL build comes from one launch per level.  Their FP32
+    entries and float orbit must be identical."""
+    import torch
+
+    dev = engine.get_device(0)
+    dev.set_stream(None)
+    td = dev.torch_device
+    rng = np.random.Generator(np.random.PCG64(ref_len))
+    # An orbit-like sequence: bounded, with |Z| spread over several octaves.
+    t = np.arange(ref_len + 1)
+    re = (0.6 * np.cos(0.37 * t) + 0.3 * rng.standard_normal(ref_len + 1) * 0.1).astype(np.float64)
+    im = (0.6 * np.sin(0.23 * t) + 0.05 * rng.standard_normal(ref_len + 1)).astype(np.float64)
+    dre, dim = torch.from_numpy(re).to(td), torch.from_numpy(im).to(td)
+    orb = _abi.ApOrbit(dre.data_ptr(), dim.data_ptr(), ref_len, 0)
+    levels = 16
+    nbytes = dev.lib.ap_bla_bytes(ref_len, levels)
+    c32, zf, end = _layout(ref_len, levels)
+    assert end == nbytes
+    tabs = {}
+    for forms in (_abi.AP_BLA_FORM_FP32, _abi.AP_BLA_FORM_ALL):
+        tab = torch.zeros(nbytes, dtype=torch.uint8, device=td)
+        dev.call("ap_bla_build", C.byref(orb), 0, 2, 0.0, 0.0, 1e-6, levels, forms, engine.ptr(tab))
+        tabs[forms] = tab
+    _sync()
+    a = to_np(tabs[_abi.AP_BLA_FORM_FP32])[c32:end]
+    b = to_np(tabs[_abi.AP_BLA_FORM_ALL])[c32:end]
+    assert np.array_equal(a, b)
+
+
+def test_empty_pixel_lists():
+    import torch
+
+    dev = engine.get_device(0)
+    td = dev.torch_device
+    rgb = torch.zeros((4, 4, 3), dtype=torch.uint8, device=td)
+    glitch = torch.ones((4, 4), dtype=torch.uint8, device=td)
+    lut_d = engine.get_gpu_cmap_lut("inferno", 2048, False, td)
+    empty = np.zeros((0, 2), np.int64)
+    for w in ("1e-5", "1e-20", "1e-250", "1e-400"):
+        exhaustive.eval_glitched_pixels_fallback_cuda(rgb, glitch, lut_d, empty, 4, 4, 100, 0, 1,
+                                                      Decimal(w), Decimal(w), 0, 0, 0.0, 0.0)
+    _sync()
+    assert int(glitch.sum().item()) == 16  # nothing touched
+    # The C entry point takes n = 0 without any launch.
+    v = _abi.ApView()
+    v.width, v.height, v.max_iter = 4, 4, 100
+    o = _abi.ApOutputs()
+    zero = torch.zeros(4, dtype=torch.int32, device=td)
+    dev.call("ap_render_limbs", C.byref(v), _abi.AP_LIMBS_GPU34, 34, None, 0, None, None,
+             engine.ptr(zero), engine.ptr(zero), C.byref(o))
+    # Compaction of an all-zero mask: count 0.
+    mask = torch.zeros(1000, dtype=torch.uint8, device=td)
+    q = torch.full((1000,), -1, dtype=torch.int32, device=td)
+    qn = torch.full((1,), -1, dtype=torch.int32, device=td)
+    dev.call("ap_compact", engine.ptr(mask), 1000, engine.ptr(q), engine.ptr(qn))
+    _sync()
+    assert int(qn.item()) == 0
+
+
+def test_oversize_frame_is_rejected_before_launch():
+    dev = engine.get_device(0)
+    v = _abi.ApView()
+    v.width, v.height, v.factor, v.max_iter = 65536, 65536, 1, 10
+    o = _abi.ApOutputs()
+    st = dev.lib.ap_render_fp64(dev.handle, C.byref(v), C.byref(o))
+    assert st == _abi.AP_ERR_ARG
+    assert b"exceeds 2^31" in dev.lib.ap_last_error(dev.handle)
+    # Limb evaluator: more limbs than it holds.
+    st = dev.lib.ap_render_limbs(dev.handle, C.byref(v), _abi.AP_LIMBS_HIGHPREC,
+                                 _abi.AP_MAX_LIMBS + 1, None, 0, None, None, None, None,
+                                 C.byref(o))
+    assert st == _abi.AP_ERR_ARG
+
+
+def test_large_frame_64bit_offsets():
+    """8192 x 8192 = 67 M pixels: counts, map and RGB land at the right
+    offsets (spot checks against the oracle's samples; the last row too)."""
+    from oracle import oracle
+
+    W = H = 8192
+    plan = engine.plan_view(W, H, 64, Decimal("-0.65"), Decimal("0.0"), Decimal("4.0"),
+                            engine_override="fp64", compute_orbit=False)
+    fr = engine.render_view(plan, "inferno", rebase=False, want_smooth=False)
+    _sync()
+    counts = fr.counts
+    rng = np.random.Generator(np.random.PCG64(8192))
+    ys = np.concatenate([rng.integers(0, H, 200), [H - 1, H - 1, 0]])
+    xs = np.concatenate([rng.integers(0, W, 200), [0, W - 1, W - 1]])
+    v = plan.view
+    for y, x in zip(ys, xs):
+        k, s, _ = oracle.fp64_sample(v, v.x_min + float(x) * v.dx, v.y_max - float(y) * v.dy)
+        assert int(counts[y, x].item()) == k
+        assert float(fr.iter_map[y, x].item()) == pytest.approx(s, abs=1e-6)
