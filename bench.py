@@ -315,6 +315,19 @@ def oracle_block(plan, orbit_host, bla_tol, c64, c32, size=48):
             "fexp32_exact": float(np.mean(blk32 == m.counts))}
 
 
+def fastmath_rate(run, seconds):
+    """Giter/s of the oracle's -O3 -ffast-math build (the "Numba fastmath"-like
+    CPU figure) over the same sample: `run(fast)` renders it once and returns
+    the iterations."""
+    total, its = 0.0, 0
+    while total < seconds or its == 0:
+        t0 = time.perf_counter()
+        its += run(True)
+        total += time.perf_counter() - t0
+    return {"fastmath_value": its / total / 1e9,
+            "fastmath": f"same sample, oracle built -O3 -ffast-math (Numba fastmath-like), {total:.1f}s"}
+
+
 def cpu_baseline_fexp(plan, seconds):
     """Oracle FloatExp (FP64 mantissa) on all host cores: a centred tile of the
     same view, grown up to the whole frame, then repeated until the sample
@@ -346,11 +359,15 @@ def cpu_baseline_fexp(plan, seconds):
         total += time.perf_counter() - t0
         iters += m.iterations
         reps += 1
-    return {"value": iters / total / 1e9, "unit": "Giter/s", "cores": threads,
-            "kind": "port", "cpu": model,
-            "sample": f"centred {tw}x{th} tile of the same view x {reps} runs, FP64-mantissa "
-                      f"FloatExp restatement with the same BLA table (oracle/apeiron_oracle.cpp, "
-                      f"OpenMP {threads} threads), {total:.1f}s"}
+    res = {"value": iters / total / 1e9, "unit": "Giter/s", "cores": threads,
+           "kind": "port", "cpu": model,
+           "sample": f"centred {tw}x{th} tile of the same view x {reps} runs, FP64-mantissa "
+                     f"FloatExp restatement with the same BLA table (oracle/apeiron_oracle.cpp, "
+                     f"OpenMP {threads} threads), {total:.1f}s"}
+    res.update(fastmath_rate(lambda fast: oracle.render("fexp64", sub, orbit=(re, im), bla=bla,
+                                                        levels=16, fast=fast).iterations,
+                             seconds / 2))
+    return res
 
 
 def run_fp64(args, torch):
@@ -409,6 +426,8 @@ def run_fp64(args, torch):
         cpu = {"value": its / total / 1e9, "unit": "Giter/s", "cores": threads, "kind": "port",
                "cpu": model, "sample": f"full {W}x{H} frame x {reps} runs, oracle FP64 "
                                        f"restatement, {total:.1f}s; counts bit-exact vs GPU: {same}"}
+        cpu.update(fastmath_rate(lambda fast: oracle.render("fp64", plan.view, fast=fast).iterations,
+                                 args.cpu_seconds / 2))
     return {
         "metric": "Giter/s per GPU, FP32+FloatExp perturb vs FP64 at matched count accuracy",
         "value": iters / wall / 1e9, "unit": "Giter/s", "ms_per_step": wall * 1e3, "dtype": "f64",
@@ -492,34 +511,45 @@ def run_flythrough(args, torch, prepared):
 def cpu_baseline_batch(plans, seconds):
     """The oracle's FP64-mantissa FloatExp restatement on all host cores over
     the fly-through's frames in turn (each a centred tile of at most 256x256),
-    until about `seconds` have run."""
+    until about `seconds` have run; then the -O3 -ffast-math build over the
+    same frames for half as long."""
     from oracle import oracle
 
     model, _ = cpu_info()
     threads = oracle.load().or_num_threads()
-    total, its, frames = 0.0, 0, 0
-    while total < seconds:
-        for p in plans:
-            v = p.view.copy()
-            re, im, ref_len = p.orbit
-            sub = v.copy()
-            tw, th = min(256, v.width), min(256, v.height)
-            x0, y0 = (v.width - tw) // 2, (v.height - th) // 2
-            sub.width, sub.height = tw, th
-            sub.ref_px, sub.ref_py = v.ref_px - x0, v.ref_py - y0
-            t0 = time.perf_counter()
-            bla = oracle.bla_build(re, im, ref_len, v.fractal_type, p.bla_tol, 16, v.gen_n,
-                                   v.gen_kr, v.gen_ki)
-            m = oracle.render("fexp64", sub, orbit=(re, im), bla=bla, levels=16)
-            total += time.perf_counter() - t0
-            its += m.iterations
-            frames += 1
-            if total >= seconds:
-                break
+
+    def frames_in_turn(limit, fast):
+        total, its, frames = 0.0, 0, 0
+        while total < limit:
+            for p in plans:
+                v = p.view.copy()
+                re, im, ref_len = p.orbit
+                sub = v.copy()
+                tw, th = min(256, v.width), min(256, v.height)
+                x0, y0 = (v.width - tw) // 2, (v.height - th) // 2
+                sub.width, sub.height = tw, th
+                sub.ref_px, sub.ref_py = v.ref_px - x0, v.ref_py - y0
+                t0 = time.perf_counter()
+                bla = oracle.bla_build(re, im, ref_len, v.fractal_type, p.bla_tol, 16, v.gen_n,
+                                       v.gen_kr, v.gen_ki)
+                m = oracle.render("fexp64", sub, orbit=(re, im), bla=bla, levels=16, fast=fast)
+                total += time.perf_counter() - t0
+                its += m.iterations
+                frames += 1
+                if total >= limit:
+                    break
+        return its, total, frames
+
+    its, total, frames = frames_in_turn(seconds, False)
+    fits, ftotal, _ = frames_in_turn(seconds / 2, True)
     return {"value": its / total / 1e9, "unit": "Giter/s", "cores": threads, "kind": "port",
             "cpu": model, "sample": f"{frames} fly-through frames in turn, centred tiles of at most "
                                     f"256x256, FP64-mantissa FloatExp restatement with its BLA table "
-                                    f"(build included), OpenMP {threads} threads, {total:.1f}s"}
+                                    f"(build included, table builder not fast-math), OpenMP "
+                                    f"{threads} threads, {total:.1f}s",
+            "fastmath_value": fits / ftotal / 1e9,
+            "fastmath": f"the same frames in turn, pass built -O3 -ffast-math (Numba "
+                        f"fastmath-like), {ftotal:.1f}s"}
 
 
 def run_copy(args, torch):
