@@ -54,6 +54,8 @@ class ApView(C.Structure):
         ("scheme_id", C.c_int32), ("is_hdr", C.c_int32),
         ("palette_offset", C.c_double), ("color_density", C.c_double),
         ("color_contrast", C.c_double),
+        ("bbsa_skip", C.c_int32), ("reserved1", C.c_int32), ("bbsa_inv_r", C.c_double),
+        ("bbsa", C.c_double * 16),
     ]
 
     def copy(self):
@@ -115,6 +117,7 @@ SIGNATURES = {
     "ap_render_fp64": (C.c_int, [_P, _VIEW, _OUT]),
     "ap_render_dd": (C.c_int, [_P, _VIEW, _OUT]),
     "ap_render_limbs": (C.c_int, [_P, _VIEW, _I, _I, _P, _I, _P, _P, _P, _P, _OUT]),
+    "ap_bbsa_coeffs": (C.c_int, [_P, _P, _I, _D, _I, _D, _I, _I, _D, _D, _I, _P]),
     "ap_render_pert64": (C.c_int, [_P, _VIEW, _ORB, _P, _OUT]),
     "ap_render_fexp64": (C.c_int, [_P, _VIEW, _ORB, _P, _OUT]),
     "ap_render_fexp32": (C.c_int, [_P, _VIEW, _ORB, _P, _OUT]),

@@ -40,7 +40,9 @@ __device__ Sample pert64_sample(const ap_view& v, const PertData& D, double si, 
     int k = 0, glitch = 0, plain = 0, nbla = 0;
     const int n = v.gen_n;
     const double kr = v.gen_kr, ki = v.gen_ki;
-    if (mandel_start<T>(v, D.zr, D.zi)) {
+    if ((T == 0 || T == 2 || T == 4) && v.bbsa_skip > 0 && bbsa_start(v, dcx, dcy, dzx, dzy)) {
+        k = v.bbsa_skip;
+    } else if (mandel_start<T>(v, D.zr, D.zi)) {
         dzx = dpx;
         dzy = dpy;
         k = plain = 1;

@@ -87,6 +87,31 @@ __device__ inline bool mandel_start(const ap_view& v, const double* zr, const do
     return (T == 0 || T == 4) && v.ref_len >= 1 && v.max_iter >= 1 && zr[0] == 0.0 && zi[0] == 0.0;
 }
 
+// Series start of the FP64-delta loop (math_core/cpu_kernels.py:388-401):
+// for |u| <= 1, u = dc / r, dz at k = skip is the polynomial A u + B u^2 + ...
+// in Horner form (cpu_eval_bbsa_4 / _8, :206-225), order 8 when any of E..H is
+// non-zero.  Returns false (start from k = 0) outside the disc.
+__device__ inline bool bbsa_start(const ap_view& v, double dcx, double dcy, double& dzx,
+                                  double& dzy) {
+    const double ux = dcx * v.bbsa_inv_r, uy = dcy * v.bbsa_inv_r;
+    if (!(ux * ux + uy * uy <= 1.0)) return false;
+    const double* c = v.bbsa;
+    bool eight = false;
+    for (int i = 8; i < 16; ++i) eight = eight || c[i] != 0.0;
+    const int top = eight ? 7 : 3;  // index of the highest coefficient
+    // t = u * coef[top]; then t = u * (coef[j] + t) down to j = 0
+    double tr = ux * c[2 * top] - uy * c[2 * top + 1];
+    double ti = ux * c[2 * top + 1] + uy * c[2 * top];
+    for (int j = top - 1; j >= 0; --j) {
+        const double sr = c[2 * j] + tr, si = c[2 * j + 1] + ti;
+        tr = ux * sr - uy * si;
+        ti = ux * si + uy * sr;
+    }
+    dzx = tr;
+    dzy = ti;
+    return true;
+}
+
 struct PertConsts {
     double inv_ref, inv_exact;
 };

@@ -75,6 +75,23 @@ class Device:
         return handle
 
 
+def has_gpu():
+    """Whether a HIP device is usable (touches the GPU runtime)."""
+    try:
+        return bool(_torch().cuda.is_available())
+    except Exception:
+        return False
+
+
+def init_primary_cuda_device():
+    """Open device 0 (init_primary_cuda_device, math_core/cuda_kernels.py:11-30);
+    True when a GPU context is up, False when there is no device."""
+    if not has_gpu():
+        return False
+    get_device(0)
+    return True
+
+
 def get_device(index=0):
     with _CTX_LOCK:
         d = _CTX.get(index)

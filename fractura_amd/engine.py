@@ -35,8 +35,8 @@ from .config import (BLA_LEVELS, GLOBAL_MAX_ITER, HISTOGRAM_BINS, HISTOGRAM_SCHE
                      compute_required_prec_bits, decimal_context)
 from .device import (clear_scratch_buffer_pool, get_device, get_scratch_bytes,
                      get_scratch_device_array, get_stream_slot, ptr)
-from .high_precision import (cache_reference_orbit, compute_single_highprec_orbit,
-                             find_optimal_reference_orbit, to_dd)
+from .high_precision import (cache_reference_orbit, compute_bbsa_coeffs,
+                             compute_single_highprec_orbit, find_optimal_reference_orbit, to_dd)
 
 __all__ = [
     "RenderParameters", "render_frame", "render_frame_ssaa", "render_frame_cpu", "recolor_frame",
@@ -80,6 +80,7 @@ class ViewPlan:
     bla_tol: float = 0.0
     orbit: tuple = None           # (ref_re, ref_im, ref_len) host arrays
     extra: dict = field(default_factory=dict)
+    bbsa: tuple = None            # (order, tol) of the series start (FP64-delta only)
 
 
 def plan_view(target_w, target_h, max_iter, center_x_dec, center_y_dec, plot_width_dec,
@@ -150,6 +151,9 @@ def plan_view(target_w, target_h, max_iter, center_x_dec, center_y_dec, plot_wid
             else:
                 v.dx, v.dy = dx_f / float(f), dy_f / float(f)
                 order, tol = int(bbsa_order), float(bbsa_tol)
+                # Orders 4 and 8 start every pixel from the series
+                # approximation (math_core/engine.py:1533-1546).
+                plan.bbsa = (order, tol)
             plan.bla = order == 32 and int(fractal_type) in (0, 2, 4) and tol > 0.0
             plan.bla_tol = tol
             if compute_orbit:
@@ -175,6 +179,42 @@ def set_reference(plan, ref_cx_dec, ref_cy_dec, u, v_frac, ref_re, ref_im, ref_l
     plan.orbit = (np.ascontiguousarray(ref_re[: ref_len + 1], dtype=np.float64),
                   np.ascontiguousarray(ref_im[: ref_len + 1], dtype=np.float64), int(ref_len))
     plan.extra["ref_cx_dec"], plan.extra["ref_cy_dec"] = ref_cx_dec, ref_cy_dec
+    set_bbsa(plan)
+
+
+def set_bbsa(plan):
+    """Series start of an FP64-delta plan for its current orbit: max_r =
+    0.75 w, no FloatExp scale (math_core/engine.py:1533-1546, 2072-2079)."""
+    v = plan.view
+    v.bbsa_skip, v.bbsa_inv_r = 0, 0.0
+    for i in range(16):
+        v.bbsa[i] = 0.0
+    if plan.engine != "pert64" or plan.bbsa is None or plan.orbit is None:
+        return
+    order, tol = plan.bbsa
+    max_r = float(plan.plot_width_dec) * 0.75
+    re, im, ref_len = plan.orbit
+    res = compute_bbsa_coeffs(re, im, ref_len, max_r, v.fractal_type, bbsa_tol=tol,
+                              bbsa_order=order, gen_n=v.gen_n, gen_kr=v.gen_kr, gen_ki=v.gen_ki,
+                              E_scale=0)
+    v.bbsa_skip = int(res[0])
+    v.bbsa_inv_r = 1.0 / max_r if max_r > 0.0 else 1.0
+    for i in range(16):
+        v.bbsa[i] = res[1 + i]
+
+
+def compute_bbsa_coeffs_cuda(d_ref_re, d_ref_im, ref_len, max_r, fractal_type, bbsa_tol=1e-4,
+                             bbsa_order=4, gen_n=3, gen_kr=0.25, gen_ki=1.0, E_scale=0, stream=0,
+                             h_ref_re=None, h_ref_im=None):
+    """The reference's entry point of the series coefficients
+    (math_core/engine.py:426-467): from host arrays when given, else from a
+    copy of the device orbit (torch tensors); the arithmetic is host-side."""
+    if h_ref_re is None or h_ref_im is None:
+        h_ref_re = d_ref_re.detach().cpu().numpy()
+        h_ref_im = d_ref_im.detach().cpu().numpy()
+    return compute_bbsa_coeffs(h_ref_re, h_ref_im, ref_len, max_r, fractal_type,
+                               bbsa_tol=bbsa_tol, bbsa_order=bbsa_order, gen_n=gen_n,
+                               gen_kr=gen_kr, gen_ki=gen_ki, E_scale=E_scale)
 
 
 @dataclass
@@ -446,7 +486,8 @@ def _rebase_loop(dev, plan, v, frame, lut, cdf, slot, glitch_mode):
                                   v.fractal_type, v.julia_cx, v.julia_cy, v.gen_n, v.gen_kr,
                                   v.gen_ki, plan.engine.startswith("fexp"))
         sub = ViewPlan(plan.engine, v.copy(), plan.center_x_dec, plan.center_y_dec,
-                       plan.plot_width_dec, plan.plot_height_dec, plan.bla, plan.bla_tol)
+                       plan.plot_width_dec, plan.plot_height_dec, plan.bla, plan.bla_tol,
+                       bbsa=plan.bbsa)
         set_reference(sub, rcx, rcy, 0.0, 0.0, re, im, rl,
                       ref_px=float(g_col) * float(f), ref_py=float(g_row) * float(f))
         orbit = _Orbit(dev, sub, slot, tag="rebase")

@@ -148,6 +148,29 @@ _PROBE_UV = [
 ]
 
 
+def compute_bbsa_coeffs(ref_re, ref_im, ref_len, max_r, fractal_type, bbsa_tol=1e-4, bbsa_order=4,
+                        gen_n=3, gen_kr=0.25, gen_ki=1.0, E_scale=0):
+    """(skip, A_re, A_im, ..., H_re, H_im): the series approximation that
+    replaces the first `skip` steps of the FP64-delta perturbation loop, in
+    u = dc / max_r (compute_bbsa_coeffs, math_core/high_precision.py:537-575).
+    Host arithmetic in libapeiron_hip.so (ap_bbsa_coeffs); no device work."""
+    from . import _abi
+
+    ref_len = int(ref_len)
+    out = np.zeros(17, dtype=np.float64)
+    re = np.ascontiguousarray(ref_re, dtype=np.float64)
+    im = np.ascontiguousarray(ref_im, dtype=np.float64)
+    if ref_len > 0 and (len(re) < ref_len or len(im) < ref_len):
+        raise ValueError("orbit shorter than ref_len")
+    lib = _abi.load()
+    st = lib.ap_bbsa_coeffs(re.ctypes.data, im.ctypes.data, ref_len, float(max_r),
+                            int(fractal_type), float(bbsa_tol), int(bbsa_order), int(gen_n),
+                            float(gen_kr), float(gen_ki), int(E_scale), out.ctypes.data)
+    if st != _abi.AP_OK:
+        raise _abi.ApError(st, "ap_bbsa_coeffs", "bad arguments")
+    return (int(out[0]),) + tuple(float(x) for x in out[1:])
+
+
 def clear_reference_orbit_cache():
     with _CACHE_LOCK:
         _CACHE.clear()

@@ -94,6 +94,15 @@ typedef struct ap_view {
     int32_t scheme_id;      /* palette transfer function 0..11              */
     int32_t is_hdr;         /* rgb is uint16 (LUT x65535) instead of uint8  */
     double palette_offset, color_density, color_contrast;
+    /* series approximation (BBSA) of the first iterations, FP64-delta
+     * perturbation only (ap_bbsa_coeffs): a pixel with |dc|/r <= 1 starts at
+     * k = bbsa_skip from dz = A u + B u^2 + ... (Horner, u = dc * bbsa_inv_r);
+     * bbsa[] = A..H as (re, im) pairs, order 8 when E..H are not all zero.
+     * bbsa_skip = 0: no series. */
+    int32_t bbsa_skip;
+    int32_t reserved1;
+    double bbsa_inv_r;
+    double bbsa[16];
 } ap_view;
 
 /*
@@ -230,6 +239,19 @@ int ap_render_limbs(ap_ctx *ctx, const ap_view *view, int32_t semantics, int32_t
                     const int32_t *queue, int32_t n, const uint32_t *coords,
                     const int32_t *signs, const uint32_t *params,
                     const int32_t *param_signs, const ap_outputs *out);
+
+/* ---- series approximation ----------------------------------------------------
+ * Replaces compute_bbsa_coeffs / _jit_compute_bbsa_coeffs
+ * (math_core/high_precision.py:51-536) and compute_bbsa_coeffs_cuda
+ * (math_core/engine.py:426-467).  Host arithmetic only (no device work, no
+ * context): from the reference orbit, the skip count and the coefficients
+ * A..H (re, im) of the polynomial in u = dc / max_r that replaces the first
+ * `skip` perturbation steps; order 4 or 8 (order 32, a FloatExp scale
+ * e_scale > 0, types other than 0/2/4 and type 4 with gen_n != 3 give skip 0).
+ * out[0] = skip, out[1..16] = A_re, A_im, ..., H_re, H_im. */
+int ap_bbsa_coeffs(const double *ref_re, const double *ref_im, int32_t ref_len, double max_r,
+                   int32_t fractal_type, double tol, int32_t order, int32_t gen_n, double gen_kr,
+                   double gen_ki, int32_t e_scale, double *out);
 
 /* ---- batched frames ------------------------------------------------------
  * No reference counterpart: the reference renders one frame per launch

@@ -278,6 +278,27 @@ Res pert64_sample(const ap_view& v, const Orbit& o, double si, double sj) {
     double ir, ie;
     inv_ln(v, ir, ie);
     int k = 0, glitch = 0;
+    // Series start, math_core/cpu_kernels.py:388-401 with cpu_eval_bbsa_4 /
+    // cpu_eval_bbsa_8 (:206-225): dz = u (a + u (b + u (c + ...))).
+    if (v.bbsa_skip > 0 && (type == 0 || type == 2 || type == 4)) {
+        const double ux = dcx * v.bbsa_inv_r, uy = dcy * v.bbsa_inv_r;
+        if (ux * ux + uy * uy <= 1.0) {
+            const double* c = v.bbsa;
+            const bool eight = c[8] != 0.0 || c[9] != 0.0 || c[10] != 0.0 || c[11] != 0.0 ||
+                               c[12] != 0.0 || c[13] != 0.0 || c[14] != 0.0 || c[15] != 0.0;
+            int j = eight ? 7 : 3;
+            double tr = ux * c[2 * j] - uy * c[2 * j + 1];
+            double ti = ux * c[2 * j + 1] + uy * c[2 * j];
+            while (j-- > 0) {
+                const double ar = c[2 * j] + tr, ai = c[2 * j + 1] + ti;
+                tr = ux * ar - uy * ai;
+                ti = ux * ai + uy * ar;
+            }
+            dzx = tr;
+            dzy = ti;
+            k = v.bbsa_skip;
+        }
+    }
     while (k < ref_len) {
         double rx = o.zr[k], ry = o.zi[k];
         double zx = rx + dzx, zy = ry + dzy;

@@ -1,0 +1,193 @@
+"""The series approximation (BBSA) of the FP64-delta perturbation start:
+coefficients from ap_bbsa_coeffs (restating _jit_compute_bbsa_coeffs,
+math_core/high_precision.py:51-536) and the per-pixel start of the oracle.
+
+Pins:
+  * closed forms on the orbit Z = 0: the series of dz' = dz^2 + r u is
+    w = (1 - sqrt(1 - 4 r u)) / 2, whose coefficients are the Catalan numbers
+    C_(j-1) r^j; of dz' = dz^3 + r u (type 4, k = 0) the ternary numbers
+    1, 1, 3, 12 on the odd powers; the Julia series r u squared at every step
+    leaves no coefficient below order 9 after four steps;
+  * the cases that disable the series (chained BLA, a FloatExp scale, types
+    1/3, type 4 with n != 3, tol <= 0, ref_len < 2);
+  * on a real orbit the polynomial reproduces the exact perturbation iterate
+    dz_skip (FP64 recurrence) within tol relative, and the skip
+    grows with tol;
+  * the reference's cap: when every step passes, skip = ref_len - 2 while the
+    coefficients are those after all ref_len steps.
+"""
+from decimal import Decimal
+
+import numpy as np
+import pytest
+
+from fractura_amd import engine
+from fractura_amd.high_precision import compute_bbsa_coeffs
+
+SEAHORSE = (Decimal("-0.743643887037158704752191506114774"),
+            Decimal("0.131825904205311970493132056385139"))
+
+
+def coeffs(res):
+    return [complex(res[1 + 2 * j], res[2 + 2 * j]) for j in range(8)]
+
+
+@pytest.mark.parametrize("order", [4, 8])
+def test_catalan_on_the_zero_orbit(order):
+    # r = 2^-7: the error ratio stays below tol at every step (order 4:
+    # 14 r^4 + 0.5 r^3), so the skip runs to the cap; at r = 1/16 order 4
+    # stops at 3 steps, as the rule says.
+    r = 2.0 ** -7
+    z = np.zeros(40)
+    res = compute_bbsa_coeffs(z, z, 39, r, 0, 1e-4, order)
+    assert res[0] == 37  # every step passes: capped at ref_len - 2
+    assert compute_bbsa_coeffs(z, z, 39, 0.0625, 0, 1e-4, 4)[0] == 3
+    catalan = [1, 1, 2, 5, 14, 42, 132, 429]
+    a = coeffs(res)
+    for j in range(order):
+        assert a[j] == catalan[j] * r ** (j + 1)
+    for j in range(order, 8):
+        assert a[j] == 0
+
+
+def test_ternary_numbers_for_the_cubic():
+    r = 0.125
+    z = np.zeros(30)
+    res = compute_bbsa_coeffs(z, z, 29, r, 4, 1e-4, 8, gen_n=3, gen_kr=0.0, gen_ki=0.0)
+    a = coeffs(res)
+    want = [r, 0, r ** 3, 0, 3 * r ** 5, 0, 12 * r ** 7, 0]
+    assert a == [complex(x) for x in want]
+
+
+def test_julia_series_vanishes_below_order_nine():
+    r = 0.25
+    z = np.zeros(20)
+    res = compute_bbsa_coeffs(z, z, 19, r, 2, 1e-4, 8)
+    assert res[0] == 17
+    assert all(c == 0 for c in coeffs(res))
+
+
+@pytest.mark.parametrize("kw", [dict(bbsa_order=32), dict(E_scale=5), dict(fractal_type=1),
+                                dict(fractal_type=3), dict(fractal_type=4, gen_n=5),
+                                dict(bbsa_tol=0.0), dict(ref_len=1), dict(max_r=0.0)])
+def test_disabled_cases(kw):
+    z = np.linspace(-0.1, 0.1, 50)
+    args = dict(ref_len=49, max_r=1e-10, fractal_type=0, bbsa_tol=1e-4, bbsa_order=4)
+    args.update(kw)
+    res = compute_bbsa_coeffs(z, z, args.pop("ref_len"), args.pop("max_r"),
+                              args.pop("fractal_type"), **args)
+    assert res == (0,) + (0.0,) * 16
+
+
+def _exact_dz(re, im, steps, dc, ftype=0, kr=0.25, ki=1.0):
+    dz = dc if ftype == 2 else 0j
+    for k in range(steps):
+        Z = complex(re[k], im[k])
+        if ftype == 4:
+            dz = (3 * Z * Z + complex(kr, ki)) * dz + 3 * Z * dz * dz + dz ** 3 + dc
+        elif ftype == 2:
+            dz = 2 * Z * dz + dz * dz
+        else:
+            dz = 2 * Z * dz + dz * dz + dc
+    return dz
+
+
+def _poly(a, u, order):
+    p = 0j
+    for j in range(order - 1, -1, -1):
+        p = u * (a[j] + p)
+    return p
+
+
+@pytest.mark.parametrize("ftype,cx,cy,w,kw", [
+    (0, *SEAHORSE, "1e-14", {}),
+    (0, *SEAHORSE, "1e-11", {}),
+    # Julia: at the repelling fixed point of z^2 - 0.12 + 0.75i.
+    (2, "1.276627768329913284189249912391588548", "-0.482856801278703605273391263782289113",
+     "1e-13", dict(julia_cx=-0.12, julia_cy=0.75)),
+    (4, "0.21", "-0.585", "1e-13", dict(gen_n=3, gen_kr=0.25, gen_ki=1.0)),
+])
+@pytest.mark.parametrize("order", [4, 8])
+def test_series_reproduces_the_perturbation_iterate(ftype, cx, cy, w, kw, order):
+    plan = engine.plan_view(96, 54, 6000, Decimal(cx), Decimal(cy), Decimal(w), fractal_type=ftype,
+                            probe=False, engine_override="pert64", **kw)
+    re, im, n = plan.orbit
+    r = float(Decimal(w)) * 0.75
+    res = compute_bbsa_coeffs(re, im, n, r, ftype, 1e-4, order, kw.get("gen_n", 3),
+                              kw.get("gen_kr", 0.25), kw.get("gen_ki", 1.0))
+    skip = res[0]
+    assert 0 < skip < n - 2, skip
+    a = coeffs(res)
+    rng = np.random.default_rng(order + 10 * ftype)
+    worst = 0.0
+    for _ in range(64):
+        u = complex(*rng.uniform(-0.7, 0.7, 2))
+        exact = _exact_dz(re, im, skip, r * u, ftype, kw.get("gen_kr", 0.25), kw.get("gen_ki", 1.0))
+        worst = max(worst, abs(_poly(a, u, order) - exact) / abs(exact))
+    # The stop rule keeps the dropped terms below tol relative to the linear
+    # one; a wrong or missing coefficient shows up at the size of dz itself.
+    assert worst < 1e-4, worst
+
+
+def test_skip_grows_with_tolerance():
+    plan = engine.plan_view(96, 54, 6000, *SEAHORSE, Decimal("1e-14"), probe=False)
+    re, im, n = plan.orbit
+    r = 0.75e-14
+    skips = [compute_bbsa_coeffs(re, im, n, r, 0, tol, 8)[0] for tol in (1e-8, 1e-6, 1e-4, 1e-2)]
+    assert skips == sorted(skips) and skips[0] < skips[-1]
+
+
+def test_capped_skip_keeps_the_last_coefficients():
+    plan = engine.plan_view(96, 54, 3000, *SEAHORSE, Decimal("1e-40"), probe=False)
+    re, im, n = plan.orbit
+    r = 0.75e-40
+    res = compute_bbsa_coeffs(re, im, n, r, 0, 1e-4, 8)
+    assert res[0] == n - 2
+    u = 0.3 + 0.2j
+    exact_n = _exact_dz(re, im, n, r * u)
+    assert abs(_poly(coeffs(res), u, 8) - exact_n) / abs(exact_n) < 1e-12
+
+
+def test_plan_carries_the_series():
+    plan = engine.plan_view(96, 54, 3000, *SEAHORSE, Decimal("1e-14"), bbsa_order=8,
+                            bbsa_tol=1e-4, probe=False)
+    assert plan.engine == "pert64"
+    v = plan.view
+    re, im, n = plan.orbit
+    res = compute_bbsa_coeffs(re, im, n, 0.75e-14, 0, 1e-4, 8)
+    assert v.bbsa_skip == res[0] > 0
+    assert v.bbsa_inv_r == 1.0 / 0.75e-14
+    assert list(v.bbsa) == list(res[1:])
+    fexp = engine.plan_view(96, 54, 3000, *SEAHORSE, Decimal("1e-14"), force_floatexp=True,
+                            probe=False)
+    assert fexp.view.bbsa_skip == 0
+    bla = engine.plan_view(96, 54, 3000, *SEAHORSE, Decimal("1e-14"), bbsa_order=32,
+                           bbsa_tol=1e-6, probe=False)
+    assert bla.view.bbsa_skip == 0
+
+
+def test_oracle_series_start_tracks_the_plain_loop():
+    """The oracle's FP64-delta loop from the series start and from k = 0 give
+    the same escape count on every pixel that neither flags as glitched."""
+    from oracle import oracle
+
+    from tests.test_gpu_pert import SERIES_VIEWS
+
+    for ftype, (cx, cy, w, kw) in SERIES_VIEWS.items():
+        plan = engine.plan_view(64, 36, 5000, Decimal(cx), Decimal(cy), Decimal(w),
+                                fractal_type=ftype, bbsa_order=8, bbsa_tol=1e-4,
+                                glitch_mode="Pauldelbrot", probe=False, engine_override="pert64",
+                                **kw)
+        re, im, _ = plan.orbit
+        v = plan.view.copy()
+        v.mode = 0
+        assert v.bbsa_skip > 0
+        with_series = oracle.render("pert64", v, orbit=(re, im))
+        v.bbsa_skip = 0
+        plain = oracle.render("pert64", v, orbit=(re, im))
+        # Pixels that neither run flags as glitched (rebasing redoes the rest)
+        # escape at the same iteration.
+        ok = (with_series.glitch == 0) & (plain.glitch == 0)
+        assert ok.any() and np.array_equal(with_series.counts[ok], plain.counts[ok]), ftype
+        assert np.mean(with_series.counts == plain.counts) >= 0.97, ftype
+        assert np.mean((plain.counts >= 0) & (plain.counts < 5000)) > 0.5, ftype

@@ -128,6 +128,35 @@ def test_pert64_types_no_bla(ftype):
     assert np.array_equal(to_np(fr.glitch), m.glitch)
 
 
+# Views where the series start skips many iterations: the seahorse, the
+# repelling fixed point of the Julia map z^2 - 0.12 + 0.75i, and a point of
+# the cubic z^3 + (0.25 + i) z + c whose orbit runs to ~1000 iterations.
+SERIES_VIEWS = {
+    0: (*SEAHORSE, "1e-14", {}),
+    2: ("1.276627768329913284189249912391588548", "-0.482856801278703605273391263782289113", "1e-13",
+        dict(julia_cx=-0.12, julia_cy=0.75)),
+    4: ("0.21", "-0.585", "1e-13", dict(gen_n=3, gen_kr=0.25, gen_ki=1.0)),
+}
+
+
+@pytest.mark.parametrize("ftype", [0, 2, 4])
+@pytest.mark.parametrize("order", [4, 8])
+def test_pert64_series_start_matches_oracle(ftype, order):
+    """The FP64-delta loop started from the series polynomial at k = skip
+    (mandelbrot_unified_perturbation_kernel with bbsa_skip > 0): counts and
+    glitch flags equal the oracle's, smooth values within 1e-9."""
+    cx, cy, w, kw = SERIES_VIEWS[ftype]
+    plan = _plan(cx, cy, w, max_iter=3000, fractal_type=ftype, bbsa_order=order, bbsa_tol=1e-4,
+                 engine_override="pert64", **kw)
+    assert not plan.bla and plan.view.bbsa_skip > 0
+    fr = _render(plan)
+    m = _oracle(plan)
+    c = to_np(fr.counts)
+    assert np.array_equal(c, m.counts), f"{np.sum(c != m.counts)} of {c.size} counts differ"
+    assert np.array_equal(to_np(fr.glitch), m.glitch)
+    assert _smooth_close(to_np(fr.smooth), m.smooth, 1e-9)
+
+
 @pytest.mark.parametrize("ftype,cx,cy", [(0, "0", "1"), (4, "0.1", "0.8"), (1, "-1.7620", "-0.0280"),
                                          (2, "0.0", "0.0")])
 def test_fexp64_matches_oracle(ftype, cx, cy):
