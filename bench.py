@@ -289,10 +289,11 @@ def run_deep(args, torch):
 
 def colour_roofline(pixels, ms):
     """The colouring pass is HBM-bound: it reads the float32 map and writes
-    RGB8, 7 bytes per pixel."""
+    RGB8, 7 bytes per pixel.  The frames use colour scheme 0, whose
+    specialisation is colorize_kernel<0>."""
     nbytes = pixels * 7
     gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    return roofline("hbm", "colorize_kernel", gbs, HBM_PEAK_GBS, "GB/s", ms, bytes=nbytes)
+    return roofline("hbm", "colorize_kernel<0>", gbs, HBM_PEAK_GBS, "GB/s", ms, bytes=nbytes)
 
 
 def oracle_block(plan, orbit_host, bla_tol, c64, c32, size=48):
