@@ -34,11 +34,16 @@ __device__ inline Sample fp64_sample(const ap_view& v, double x, double y) {
     const int max_iter = v.max_iter;
     const double kr = v.gen_kr, ki = v.gen_ki;
     const int n = v.gen_n;
-    for (int k = 0; k < max_iter; ++k) {
+    // One exit for both ends (escape at k < max_iter, or k = max_iter): the
+    // escape is re-tested after the loop from the same z, so the iterations
+    // carry nothing for the exit paths.  At k = max_iter |z|^2 is formed and
+    // not tested, as the reference returns there without testing it.
+    int k = 0;
+    for (;; ++k) {
         double zx2 = zx * zx;
         double zy2 = zy * zy;
         double mag2 = zx2 + zy2;
-        if (mag2 > kBailout2) return escaped(k, mag2, inv_ref, inv_exact, v.ln_bailout_log);
+        if (k >= max_iter || mag2 > kBailout2) break;
         if (T == AP_MANDELBROT || T == AP_JULIA_MANDELBROT) {
             zy = 2.0 * zx * zy + cy;
             zx = zx2 - zy2 + cx;
@@ -58,6 +63,7 @@ __device__ inline Sample fp64_sample(const ap_view& v, double x, double y) {
             zy = pi + kzi + cy;
         }
     }
+    if (k < max_iter) return escaped(k, zx * zx + zy * zy, inv_ref, inv_exact, v.ln_bailout_log);
     return interior(max_iter, max_iter);
 }
 

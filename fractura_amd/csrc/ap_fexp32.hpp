@@ -99,17 +99,81 @@ __device__ inline int bla_leap32(const Fexp32Data& D, float& dzx, float& dzy, in
     return d;
 }
 
+// True when pred holds on every active lane of the wave.
+__device__ inline bool wave_all(bool pred) {
+    return __builtin_amdgcn_ballot_w64(pred) == __builtin_amdgcn_read_exec();
+}
+
+// An entry read through the scalar cache: idx must be wave-uniform (the
+// load is then one s_load_dwordx8 into scalar registers, and the leap's
+// products take their coefficients from there).  Read as one 8-dword vector
+// so the whole entry arrives in a single load.
+typedef int Int8 __attribute__((ext_vector_type(8)));
+typedef const __attribute__((address_space(4))) Int8 ScalarInt8;
+__device__ inline Coef32 scalar_entry(const Coef32* base, unsigned idx) {
+    const Int8 w = ((ScalarInt8*)base)[idx];
+    Coef32 c;
+    c.ar = __int_as_float(w[0]);
+    c.ai = __int_as_float(w[1]);
+    c.br = __int_as_float(w[2]);
+    c.bi = __int_as_float(w[3]);
+    c.ae = w[4];
+    c.be = w[5];
+    c.key = w[6];
+    c.next_hi = w[7];
+    return c;
+}
+
+// Wave-uniform run of deep leaps.  In deep views the lanes of a wave share
+// k and the level they may take (|dz| is far below every radius), so the
+// entry, the iteration index and the level are wave-uniform: they stay in
+// scalar registers and only the delta arithmetic runs per lane.  The run
+// takes the leap at level hi from k while that level is valid on every lane
+// and every lane is still deep; it stops at the first leap that is not, which
+// the per-lane loop then takes as it would have.  The arithmetic is that of
+// the per-lane leap, so the results are identical.
+template <bool kNoDc>
+__device__ inline void uniform_leaps32(const Fexp32Data& D, float& dzx, float& dzy, int& z_exp,
+                                       float dpx, float dpy, int& k, int& hi, int& nbla, int E,
+                                       int z_deep) {
+    int ku = __builtin_amdgcn_readfirstlane(k);
+    int hu = __builtin_amdgcn_readfirstlane(hi);
+    if (!wave_all(k == ku && hi == hu)) return;
+    int steps = 0;
+    for (;;) {
+        const Coef32 c = scalar_entry(D.bla.c32, D.bla.at(ku, hu));
+        const float q = fmaf(dzx, dzx, dzy * dzy);
+        if (!wave_all(dz_key(q, z_exp - E) < c.key)) break;
+        float rx, ry;
+        int ne;
+        leap32_terms<kNoDc>(c, q, dzx, dzy, z_exp, dpx, dpy, rx, ry, ne);
+        dzx = rx;
+        dzy = ry;
+        z_exp = ne;
+        ku += 1 << hu;
+        hu = c.next_hi;
+        ++steps;
+        if (hu < 0 || !wave_all(z_exp < z_deep)) break;
+    }
+    k = ku;
+    hi = hu;
+    nbla += steps;
+}
+
 // Consecutive leaps while z_exp < z_deep, in a loop with a single exit and
 // a single (rare) branch, so the exec-mask bookkeeping per leap stays small:
 // a lane whose level 0 is not valid at its k applies the identity entry
 // (A = 1, B = 0, which rescales its mantissa exactly) with next_hi = -2, and
-// leaves the loop; the -2 then sets no_leap.
+// leaves the loop; the -2 then sets no_leap.  Each pass first tries a
+// wave-uniform run (uniform_leaps32).
 template <bool kNoDc>
 __device__ inline void deep_leaps32(const Fexp32Data& D, float& dzx, float& dzy, int& z_exp,
                                     float dpx, float dpy, int& k, int& nbla, bool& no_leap,
                                     int E, int z_deep, int ref_len) {
     int hi = top_level(k, ref_len, D.bla.levels);
     while (z_exp < z_deep && hi >= 0) {
+        uniform_leaps32<kNoDc>(D, dzx, dzy, z_exp, dpx, dpy, k, hi, nbla, E, z_deep);
+        if (!(z_exp < z_deep && hi >= 0)) break;
         Coef32 c = entry(D.bla.c32, D.bla.at(k, hi));
         const float q = fmaf(dzx, dzx, dzy * dzy);
         const int key = dz_key(q, z_exp - E);
@@ -150,74 +214,88 @@ __device__ inline void deep_leaps32(const Fexp32Data& D, float& dzx, float& dzy,
 // with BLA to the end.)
 constexpr int kTail32 = -32;
 
+// One FP64 perturbation step of the tail.  kCubic: type 4 with n = 3, whose
+// step has a loop of its own (a runtime n inside the loop leaves the
+// compiler shuffling registers between the two forms every iteration).
+template <int T, bool kCubic>
+__device__ inline void tail_step(const ap_view& v, double rx, double ry, double& dzx, double& dzy,
+                                 double dpx, double dpy) {
+    if (T == 0 || T == 2) {
+        double ndx = (2.0 * rx + dzx) * dzx - (2.0 * ry + dzy) * dzy + dpx;
+        dzy = 2.0 * ((rx + dzx) * dzy + ry * dzx) + dpy;
+        dzx = ndx;
+    } else if (T == 1 || T == 3) {
+        double ndx = (2.0 * rx + dzx) * dzx - (2.0 * ry + dzy) * dzy + dpx;
+        double sxs = rx >= 0.0 ? 1.0 : -1.0;
+        double sys = ry >= 0.0 ? 1.0 : -1.0;
+        double ddx = (rx + dzx) * sxs >= 0.0 ? sxs * dzx : -sxs * (2.0 * rx + dzx);
+        double ddy = (ry + dzy) * sys >= 0.0 ? sys * dzy : -sys * (2.0 * ry + dzy);
+        dzy = 2.0 * (fabs(rx) * ddy + fabs(ry) * ddx + ddx * ddy) - dpy;
+        dzx = ndx;
+    } else if (kCubic) {
+        // (Z+d)^3 - Z^3 + k d = d ((3 Z^2 + k) + d (3 Z + d)), in FMA form.
+        const double ux = fma(3.0, rx, dzx), uy = fma(3.0, ry, dzy);
+        const double ax = fma(3.0, fma(rx, rx, -(ry * ry)), v.gen_kr);
+        const double ay = fma(3.0, 2.0 * rx * ry, v.gen_ki);
+        const double wx = fma(dzx, ux, fma(-dzy, uy, ax));
+        const double wy = fma(dzx, uy, fma(dzy, ux, ay));
+        const double ndx = fma(dzx, wx, fma(-dzy, wy, dpx));
+        dzy = fma(dzx, wy, fma(dzy, wx, dpy));
+        dzx = ndx;
+    } else {
+        const int n = v.gen_n;
+        double pr = dzx, pi = dzy, qr = rx, qi = ry;
+        for (int e = 0; e < n - 1; ++e) {
+            double t1r = pr * rx - pi * ry, t1i = pr * ry + pi * rx;
+            double sr = pr + qr, sim = pi + qi;
+            double t2r = sr * dzx - sim * dzy, t2i = sr * dzy + sim * dzx;
+            pr = t1r + t2r;
+            pi = t1i + t2i;
+            double nq = qr * rx - qi * ry;
+            qi = qr * ry + qi * rx;
+            qr = nq;
+        }
+        double kdr = v.gen_kr * dzx - v.gen_ki * dzy, kdi = v.gen_kr * dzy + v.gen_ki * dzx;
+        dzx = pr + kdr + dpx;
+        dzy = pi + kdi + dpy;
+    }
+}
+
 // FP64 perturbation continuation of the FP32 loop from iteration k, with the
 // delta now in actual units (dc * 2^-E underflows to 0 beyond FP64 range,
 // where it is negligible next to |dz| >= 2^kTail32).
-template <int T>
-__device__ Sample fexp32_tail(const ap_view& v, const Fexp32Data& D, const PertConsts& pc,
-                              double dzx, double dzy, double dcx_d, double dcy_d, int k,
-                              int glitch, int plain, int nbla) {
+template <int T, bool kCubic>
+__device__ Sample fexp32_tail_loop(const ap_view& v, const Fexp32Data& D, const PertConsts& pc,
+                                   double dzx, double dzy, double dcx_d, double dcy_d, int k,
+                                   int glitch, int plain, int nbla) {
     const bool julia = (T == 2 || T == 3);
     const int ref_len = v.ref_len;
     const bool strict = v.strict_glitch > 0;
     const double sc_e = v.e_scale <= 1074 ? ldexp(1.0, -v.e_scale) : 0.0;
     const double dpx = julia ? 0.0 : dcx_d * sc_e, dpy = julia ? 0.0 : dcy_d * sc_e;
-    const int n = v.gen_n;
+    // One exit for both ends of the loop (escape at k, or k = ref_len): the
+    // escape is re-tested after it from the same operands, so no value of the
+    // escape path has to be carried through the iterations.
     while (k < ref_len) {
         const double rx = D.zr[k], ry = D.zi[k];
         const double zx = rx + dzx, zy = ry + dzy;
         const double mag2 = zx * zx + zy * zy;
-        if (mag2 > kBailout2) {
-            Sample r = escaped(k, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
-            r.glitch = glitch;
-            r.plain = plain;
-            r.bla = nbla;
-            return r;
-        }
+        if (mag2 > kBailout2) break;
         if (strict && k > 15) {
             double r_mag2 = rx * rx + ry * ry;
             if (mag2 < 1e-6 * r_mag2 && r_mag2 > 1e-8) glitch = 1;
         }
-        if (T == 0 || T == 2) {
-            double ndx = (2.0 * rx + dzx) * dzx - (2.0 * ry + dzy) * dzy + dpx;
-            dzy = 2.0 * ((rx + dzx) * dzy + ry * dzx) + dpy;
-            dzx = ndx;
-        } else if (T == 1 || T == 3) {
-            double ndx = (2.0 * rx + dzx) * dzx - (2.0 * ry + dzy) * dzy + dpx;
-            double sxs = rx >= 0.0 ? 1.0 : -1.0;
-            double sys = ry >= 0.0 ? 1.0 : -1.0;
-            double ddx = (rx + dzx) * sxs >= 0.0 ? sxs * dzx : -sxs * (2.0 * rx + dzx);
-            double ddy = (ry + dzy) * sys >= 0.0 ? sys * dzy : -sys * (2.0 * ry + dzy);
-            dzy = 2.0 * (fabs(rx) * ddy + fabs(ry) * ddx + ddx * ddy) - dpy;
-            dzx = ndx;
-        } else if (n == 3) {
-            // (Z+d)^3 - Z^3 + k d = d ((3 Z^2 + k) + d (3 Z + d)), in FMA form.
-            const double ux = fma(3.0, rx, dzx), uy = fma(3.0, ry, dzy);
-            const double ax = fma(3.0, fma(rx, rx, -(ry * ry)), v.gen_kr);
-            const double ay = fma(3.0, 2.0 * rx * ry, v.gen_ki);
-            const double wx = fma(dzx, ux, fma(-dzy, uy, ax));
-            const double wy = fma(dzx, uy, fma(dzy, ux, ay));
-            const double ndx = fma(dzx, wx, fma(-dzy, wy, dpx));
-            dzy = fma(dzx, wy, fma(dzy, wx, dpy));
-            dzx = ndx;
-        } else {
-            double pr = dzx, pi = dzy, qr = rx, qi = ry;
-            for (int e = 0; e < n - 1; ++e) {
-                double t1r = pr * rx - pi * ry, t1i = pr * ry + pi * rx;
-                double sr = pr + qr, sim = pi + qi;
-                double t2r = sr * dzx - sim * dzy, t2i = sr * dzy + sim * dzx;
-                pr = t1r + t2r;
-                pi = t1i + t2i;
-                double nq = qr * rx - qi * ry;
-                qi = qr * ry + qi * rx;
-                qr = nq;
-            }
-            double kdr = v.gen_kr * dzx - v.gen_ki * dzy, kdi = v.gen_kr * dzy + v.gen_ki * dzx;
-            dzx = pr + kdr + dpx;
-            dzy = pi + kdi + dpy;
-        }
+        tail_step<T, kCubic>(v, rx, ry, dzx, dzy, dpx, dpy);
         ++k;
         ++plain;
+    }
+    if (k < ref_len) {
+        const double zx = D.zr[k] + dzx, zy = D.zi[k] + dzy;
+        Sample r = escaped(k, zx * zx + zy * zy, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
+        r.glitch = glitch;
+        r.plain = plain;
+        r.bla = nbla;
+        return r;
     }
     const double zx = D.zr[ref_len] + dzx, zy = D.zi[ref_len] + dzy;
     const double scfb = v.e_scale <= 1020 ? pow2_f64(-v.e_scale) : 0.0;
@@ -233,6 +311,15 @@ __device__ Sample fexp32_tail(const ap_view& v, const Fexp32Data& D, const PertC
     }
     if (ref_len < v.max_iter) glitch = 1;
     return stage2<T>(v, pc, zx, zy, cx, cy, k, glitch, plain, nbla);
+}
+
+template <int T>
+__device__ Sample fexp32_tail(const ap_view& v, const Fexp32Data& D, const PertConsts& pc,
+                              double dzx, double dzy, double dcx_d, double dcy_d, int k,
+                              int glitch, int plain, int nbla) {
+    if (T == 4 && v.gen_n == 3)
+        return fexp32_tail_loop<T, true>(v, D, pc, dzx, dzy, dcx_d, dcy_d, k, glitch, plain, nbla);
+    return fexp32_tail_loop<T, false>(v, D, pc, dzx, dzy, dcx_d, dcy_d, k, glitch, plain, nbla);
 }
 
 template <int T>

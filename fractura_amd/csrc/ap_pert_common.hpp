@@ -133,22 +133,14 @@ __device__ inline Sample stage2(const ap_view& v, const PertConsts& pc, double z
                                 double cx, double cy, int k, int glitch, int plain, int bla) {
     const int max_iter = v.max_iter;
     const int n = v.gen_n;
+    // One exit for a non-finite or escaped |z|^2 (mag2 is a sum of squares,
+    // so "not <= bailout" is exactly NaN, +inf or above it); which of them it
+    // was is decided after the loop from the same z, so the iterations carry
+    // nothing for the exit paths.
     while (k < max_iter) {
         double zx2 = zx * zx, zy2 = zy * zy;
         double mag2 = zx2 + zy2;
-        if (isnan(mag2) || isinf(mag2)) {
-            Sample r = nan_exit(k);
-            r.plain = plain;
-            r.bla = bla;
-            return r;
-        }
-        if (mag2 > kBailout2) {
-            Sample r = escaped(k, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
-            r.glitch = glitch;
-            r.plain = plain;
-            r.bla = bla;
-            return r;
-        }
+        if (!(mag2 <= kBailout2)) break;
         if (T == 0 || T == 2) {
             zy = 2.0 * zx * zy + cy;
             zx = zx2 - zy2 + cx;
@@ -169,6 +161,20 @@ __device__ inline Sample stage2(const ap_view& v, const PertConsts& pc, double z
         }
         ++k;
         ++plain;
+    }
+    if (k < max_iter) {
+        const double mag2 = zx * zx + zy * zy;
+        if (isnan(mag2) || isinf(mag2)) {
+            Sample r = nan_exit(k);
+            r.plain = plain;
+            r.bla = bla;
+            return r;
+        }
+        Sample r = escaped(k, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
+        r.glitch = glitch;
+        r.plain = plain;
+        r.bla = bla;
+        return r;
     }
     Sample r = interior(max_iter, max_iter);
     r.glitch = glitch;
