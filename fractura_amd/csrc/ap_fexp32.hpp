@@ -313,16 +313,9 @@ __device__ Sample fexp32_tail_loop(const ap_view& v, const Fexp32Data& D, const 
     return stage2<T>(v, pc, zx, zy, cx, cy, k, glitch, plain, nbla);
 }
 
-template <int T>
-__device__ Sample fexp32_tail(const ap_view& v, const Fexp32Data& D, const PertConsts& pc,
-                              double dzx, double dzy, double dcx_d, double dcy_d, int k,
-                              int glitch, int plain, int nbla) {
-    if (T == 4 && v.gen_n == 3)
-        return fexp32_tail_loop<T, true>(v, D, pc, dzx, dzy, dcx_d, dcy_d, k, glitch, plain, nbla);
-    return fexp32_tail_loop<T, false>(v, D, pc, dzx, dzy, dcx_d, dcy_d, k, glitch, plain, nbla);
-}
-
-template <int T>
+// kCubic: type 4 with n = 3 (the Horner/FMA step); the dispatch on n happens
+// once per sample, outside the loops.
+template <int T, bool kCubic>
 __device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si, double sj) {
     const bool julia = (T == 2 || T == 3);
     const double dcy_d = (v.ref_py - sj) * v.dy;
@@ -360,12 +353,11 @@ __device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si
             if (k >= ref_len) break;
         }
         const int eff = z_exp - E;
-        // |dz| = max|dz_m| 2^eff >= 2^kTail32: continue in FP64.
+        // |dz| = max|dz_m| 2^eff >= 2^kTail32: continue in FP64 (the same
+        // hand-off as at the end of the orbit, after the loop).
         if (eff >= kTail32 - 40) {
             float m = fmaxf(fabsf(dzx), fabsf(dzy));
-            if (m > 0.0f && eff + ilogbf(m) >= kTail32)
-                return fexp32_tail<T>(v, D, pc, ldexp((double)dzx, eff), ldexp((double)dzy, eff),
-                                      dcx_d, dcy_d, k, glitch, plain, nbla);
+            if (m > 0.0f && eff + ilogbf(m) >= kTail32) break;
         }
         const float2 Z = D.zf[k];
         if (strict && k > 15 && eff >= -40) {
@@ -409,7 +401,7 @@ __device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si
             float ly = 2.0f * (fabsf(Z.x) * ddy + fabsf(Z.y) * ddx + ddx * ddy * s_nl) - cys;
             dzx = lx;
             dzy = ly;
-        } else if (n == 3) {
+        } else if (kCubic) {
             // ((Z+d)^3 - Z^3 + k d) / s = dz ((3 Z^2 + k) + d (3 Z + d)), d = dz s,
             // s = 2^(z_exp-E), in FMA form.
             const float tx = dzx * s_nl, ty = dzy * s_nl;
@@ -441,10 +433,11 @@ __device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si
         ++k;
         ++plain;
     }
-    // The reference escaped with the delta still tiny: Stage 2 from
-    // Z_ref_len + dz in FP64.
-    return fexp32_tail<T>(v, D, pc, ldexp((double)dzx, z_exp - E), ldexp((double)dzy, z_exp - E),
-                          dcx_d, dcy_d, k, glitch, plain, nbla);
+    // The delta reached 2^kTail32, or the orbit ended with it still tiny (the
+    // tail then goes straight to Stage 2 from Z_ref_len + dz): FP64 from k.
+    return fexp32_tail_loop<T, kCubic>(v, D, pc, ldexp((double)dzx, z_exp - E),
+                                       ldexp((double)dzy, z_exp - E), dcx_d, dcy_d, k, glitch,
+                                       plain, nbla);
 }
 
 template <int T>
@@ -452,7 +445,8 @@ struct EngFexp32 {
     using Data = Fexp32Data;
     static constexpr bool kIterInSubsamples = true;
     __device__ static Sample sample(const ap_view& v, const Data& d, double si, double sj) {
-        return fexp32_sample<T>(v, d, si, sj);
+        if (T == 4 && v.gen_n == 3) return fexp32_sample<T, true>(v, d, si, sj);
+        return fexp32_sample<T, false>(v, d, si, sj);
     }
 };
 
