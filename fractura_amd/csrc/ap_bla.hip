@@ -154,6 +154,27 @@ __device__ inline BlaEntry bla_merge(const BlaEntry& e1, const BlaEntry& e2) {
     return out;
 }
 
+// A block's entries in LDS, one array per field: a wave's 8-byte accesses to
+// consecutive entries are free of bank conflicts (an array of double4 would
+// put 32 bytes between lanes).
+template <int N>
+struct LdsEntries {
+    double ar[N], ai[N], br[N], bi[N], r2[N];
+    __device__ BlaEntry get(int i) const {
+        BlaEntry e;
+        e.c = make_double4(ar[i], ai[i], br[i], bi[i]);
+        e.r2 = r2[i];
+        return e;
+    }
+    __device__ void put(int i, const BlaEntry& e) {
+        ar[i] = e.c.x;
+        ai[i] = e.c.y;
+        br[i] = e.c.z;
+        bi[i] = e.c.w;
+        r2[i] = e.r2;
+    }
+};
+
 __device__ inline Coef32 make_coef32(double4 c, double r2, int next_hi) {
     Coef32 q;
     double am = fmax(fabs(c.x), fabs(c.y));
@@ -193,19 +214,18 @@ constexpr int kLowPer = (kLowSpan + kLowThreads - 1) / kLowThreads;
 // kLowOut.  Level d is needed on [k0, k0 + kLowOut + 2^top - 2^d): each merge
 // reads its own index and the one 2^(d-1) to the right, so every entry of
 // level d-1 is read before any of level d is written (one barrier between the
-// phases).  sc/sr: the block's LDS, kLowSpan entries each.
+// phases).  lds: the block's LDS, kLowSpan entries.
 __device__ inline void bla_low_block(const BlaMut& t, const double* __restrict__ zr,
                                      const double* __restrict__ zi, int type, int n, double kr,
-                                     double ki, double tol, int top, int blk, double4* sc,
-                                     double* sr) {
+                                     double ki, double tol, int top, int blk,
+                                     LdsEntries<kLowSpan>& lds) {
     const int k0 = blk * kLowOut;
     const int ref_len = t.ref_len;
     const int span0 = kLowOut + (1 << top) - 1;
     for (int i = threadIdx.x; i < span0; i += kLowThreads) {
         const int k = k0 + i;
         BlaEntry e = k < ref_len ? bla_level0(zr, zi, k, ref_len, type, n, kr, ki, tol) : bla_zero();
-        sc[i] = e.c;
-        sr[i] = e.r2;
+        lds.put(i, e);
         if (i < kLowOut && k < ref_len) store_entry(t, k, 0, e);
         if (i < kLowOut && k <= ref_len && (t.forms & AP_BLA_FORM_FP32))
             t.zf[k] = make_float2((float)zr[k], (float)zi[k]);
@@ -219,22 +239,14 @@ __device__ inline void bla_low_block(const BlaMut& t, const double* __restrict__
         for (int j = 0; j < kLowPer; ++j) {
             const int i = threadIdx.x + j * kLowThreads;
             v[j] = bla_zero();
-            if (i < span && k0 + i < ref_len - 2 * step) {
-                BlaEntry e1, e2;
-                e1.c = sc[i];
-                e1.r2 = sr[i];
-                e2.c = sc[i + step];
-                e2.r2 = sr[i + step];
-                v[j] = bla_merge(e1, e2);
-            }
+            if (i < span && k0 + i < ref_len - 2 * step) v[j] = bla_merge(lds.get(i), lds.get(i + step));
         }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kLowPer; ++j) {
             const int i = threadIdx.x + j * kLowThreads;
             if (i < span) {
-                sc[i] = v[j].c;
-                sr[i] = v[j].r2;
+                lds.put(i, v[j]);
                 if (i < kLowOut && k0 + i < ref_len) store_entry(t, k0 + i, d, v[j]);
             }
         }
@@ -246,9 +258,8 @@ __global__ __launch_bounds__(kLowThreads) void bla_low_kernel(BlaMut t,
                                                               const double* __restrict__ zi,
                                                               int type, int n, double kr,
                                                               double ki, double tol, int top) {
-    __shared__ double4 sc[kLowSpan];
-    __shared__ double sr[kLowSpan];
-    bla_low_block(t, zr, zi, type, n, kr, ki, tol, top, blockIdx.x, sc, sr);
+    __shared__ LdsEntries<kLowSpan> lds;
+    bla_low_block(t, zr, zi, type, n, kr, ki, tol, top, blockIdx.x, lds);
 }
 
 // Batched tables: block b belongs to the last job whose low_begin <= b
@@ -268,12 +279,11 @@ __device__ inline int job_of(const ap_bla_job* jobs, int njobs, long long x, boo
 __global__ __launch_bounds__(kLowThreads) void bla_low_batch_kernel(const ap_bla_job* __restrict__ jobs,
                                                                     int njobs, int levels, int forms,
                                                                     int top) {
-    __shared__ double4 sc[kLowSpan];
-    __shared__ double sr[kLowSpan];
+    __shared__ LdsEntries<kLowSpan> lds;
     const ap_bla_job& J = jobs[job_of(jobs, njobs, blockIdx.x, false)];
     const BlaMut t = bla_mut(J.table, J.orbit.len, levels, forms, top, levels - 1);
     bla_low_block(t, J.orbit.re, J.orbit.im, J.fractal_type, J.gen_n, J.gen_kr, J.gen_ki, J.tol,
-                  top, blockIdx.x - J.low_begin, sc, sr);
+                  top, blockIdx.x - J.low_begin, lds);
 }
 
 constexpr int kHighRes = 4;                     // adjacent residue classes per block
@@ -292,8 +302,7 @@ constexpr int kHighPer = (kHighRes * kHighSpan + kHighThreads - 1) / kHighThread
 // by no output of the block (or lies past ref_len - 2^d, hence is zero), so
 // it is computed as zero.
 __global__ __launch_bounds__(kHighThreads) void bla_high_kernel(BlaMut t, int top) {
-    __shared__ double4 sc[kHighRes * kHighSpan];
-    __shared__ double sr[kHighRes * kHighSpan];
+    __shared__ LdsEntries<kHighRes * kHighSpan> lds;
     const int R = 1 << top;
     const int groups = R / kHighRes;
     const int r0 = (int)(blockIdx.x % (unsigned)groups) * kHighRes;
@@ -310,8 +319,10 @@ __global__ __launch_bounds__(kHighThreads) void bla_high_kernel(BlaMut t, int to
         const int k = r + (m0 + i) * R;
         if (k < ref_len) {
             const size_t g = (size_t)top * ref_len + (size_t)k;
-            sc[q] = t.coef[g];
-            sr[q] = t.r2[g];
+            BlaEntry e;
+            e.c = t.coef[g];
+            e.r2 = t.r2[g];
+            lds.put(q, e);
         }
     }
     for (int d = top + 1; d < t.levels; ++d) {
@@ -323,14 +334,8 @@ __global__ __launch_bounds__(kHighThreads) void bla_high_kernel(BlaMut t, int to
             const int q = threadIdx.x + j * kHighThreads;
             const int k = r0 + (q % kHighRes) + (m0 + q / kHighRes) * R;
             v[j] = bla_zero();
-            if (q < items && q + sm < items && k < ref_len - (1 << d)) {
-                BlaEntry e1, e2;
-                e1.c = sc[q];
-                e1.r2 = sr[q];
-                e2.c = sc[q + sm];
-                e2.r2 = sr[q + sm];
-                v[j] = bla_merge(e1, e2);
-            }
+            if (q < items && q + sm < items && k < ref_len - (1 << d))
+                v[j] = bla_merge(lds.get(q), lds.get(q + sm));
         }
         __syncthreads();
 #pragma unroll
@@ -339,8 +344,7 @@ __global__ __launch_bounds__(kHighThreads) void bla_high_kernel(BlaMut t, int to
             if (q < items) {
                 const int i = q / kHighRes;
                 const int k = r0 + (q % kHighRes) + (m0 + i) * R;
-                sc[q] = v[j].c;
-                sr[q] = v[j].r2;
+                lds.put(q, v[j]);
                 if (i < kHighOut && k < ref_len) store_entry(t, k, d, v[j]);
             }
         }
