@@ -94,11 +94,15 @@ __device__ inline Sample dd_sample(const ap_view& v, dd cx_in, dd cy_in) {
     inv_ln_pair(v, inv_ref, inv_exact);
     const int max_iter = v.max_iter;
     const int n = v.gen_n;
-    for (int k = 0; k < max_iter; ++k) {
+    // One exit for both ends, as in the FP64 loop (the escape test is on the
+    // high words only, math_core/cpu_kernels.py:292-295).
+    int k = 0;
+    double mag2;
+    for (;; ++k) {
         dd zx2 = dd_sqr(zx);
         dd zy2 = dd_sqr(zy);
-        double mag2 = zx2.hi + zy2.hi;
-        if (mag2 > kBailout2) return escaped(k, mag2, inv_ref, inv_exact, v.ln_bailout_log);
+        mag2 = zx2.hi + zy2.hi;
+        if (k >= max_iter || mag2 > kBailout2) break;
         if (T == AP_MANDELBROT || T == AP_JULIA_MANDELBROT) {
             dd sub = dd_sub(zx2, zy2);
             dd zxy = dd_mul(zx, zy);
@@ -123,6 +127,7 @@ __device__ inline Sample dd_sample(const ap_view& v, dd cx_in, dd cy_in) {
             zy = dd_add(dd_add(pi, kzi), cy);
         }
     }
+    if (k < max_iter) return escaped(k, mag2, inv_ref, inv_exact, v.ln_bailout_log);
     return interior(max_iter, max_iter);
 }
 

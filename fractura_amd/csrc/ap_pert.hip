@@ -48,18 +48,15 @@ __device__ Sample pert64_sample(const ap_view& v, const PertData& D, double si, 
         k = plain = 1;
     }
 
+    // One exit for every way out (escape at k, a non-finite leap at k, or
+    // k = ref_len); the exit is told apart after the loop, so the iterations
+    // carry nothing for the exit paths.
+    bool leap_nan = false;
     while (k < ref_len) {
         const double rx = D.zr[k], ry = D.zi[k];
         const double zx = rx + dzx, zy = ry + dzy;
         const double mag2 = zx * zx + zy * zy;
-        if (mag2 > kBailout2) {
-            if (strict && k > 15 && (rx * rx + ry * ry) < 4.0) glitch = 1;
-            Sample r = escaped(k, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
-            r.glitch = glitch;
-            r.plain = plain;
-            r.bla = nbla;
-            return r;
-        }
+        if (mag2 > kBailout2) break;
         if (strict && k > 15) {
             double r_mag2 = rx * rx + ry * ry;
             if (mag2 < 1e-6 * r_mag2 && r_mag2 > 1e-8) glitch = 1;
@@ -91,10 +88,8 @@ __device__ Sample pert64_sample(const ap_view& v, const PertData& D, double si, 
                 dzx = ndx;
                 ++nbla;
                 if (isnan(dzx) || isnan(dzy) || isinf(dzx) || isinf(dzy)) {
-                    Sample r = nan_exit(k);
-                    r.plain = plain;
-                    r.bla = nbla;
-                    return r;
+                    leap_nan = true;
+                    break;
                 }
                 k += 1 << best;
                 continue;
@@ -131,6 +126,21 @@ __device__ Sample pert64_sample(const ap_view& v, const PertData& D, double si, 
         }
         ++k;
         ++plain;
+    }
+    if (k < ref_len) {
+        Sample r;
+        if (leap_nan) {
+            r = nan_exit(k);
+        } else {
+            const double rx = D.zr[k], ry = D.zi[k];
+            const double zx = rx + dzx, zy = ry + dzy;
+            if (strict && k > 15 && (rx * rx + ry * ry) < 4.0) glitch = 1;
+            r = escaped(k, zx * zx + zy * zy, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
+            r.glitch = glitch;
+        }
+        r.plain = plain;
+        r.bla = nbla;
+        return r;
     }
     const double zx = D.zr[ref_len] + dzx, zy = D.zi[ref_len] + dzy;
     const double cx = julia ? v.julia_cx : v.ref_cx + dcx;
@@ -221,6 +231,9 @@ __device__ Sample fexp64_sample(const ap_view& v, const PertData& D, double si, 
         k = plain = 1;
     }
 
+    // One exit for every way out (escape at k, a non-finite leap at k, or
+    // k = ref_len), told apart after the loop.
+    bool leap_nan = false;
     while (k < ref_len) {
         // Deep regime (eff < -50): the reference's escape test (eff >= -50) and
         // the Pauldelbrot test (returns 0 below eff = -40) are both skipped, so
@@ -235,15 +248,13 @@ __device__ Sample fexp64_sample(const ap_view& v, const PertData& D, double si, 
                 }
                 ++nbla;
                 if (isnan(dzx) || isnan(dzy) || isinf(dzx) || isinf(dzy)) {
-                    Sample r = nan_exit(k);
-                    r.plain = plain;
-                    r.bla = nbla;
-                    return r;
+                    leap_nan = true;
+                    break;
                 }
                 renorm64(dzx, dzy, z_exp);
                 k += step;
             }
-            if (k >= ref_len) break;
+            if (leap_nan || k >= ref_len) break;
         }
         const double rx = D.zr[k], ry = D.zi[k];
         const int eff = z_exp - E;
@@ -251,13 +262,7 @@ __device__ Sample fexp64_sample(const ap_view& v, const PertData& D, double si, 
             double sc = pow2_f64(eff);
             double zx = rx + dzx * sc, zy = ry + dzy * sc;
             double mag2 = zx * zx + zy * zy;
-            if (mag2 > kBailout2) {
-                Sample r = escaped(k, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
-                r.glitch = glitch;
-                r.plain = plain;
-                r.bla = nbla;
-                return r;
-            }
+            if (mag2 > kBailout2) break;
         }
         // With max(|dz_m|) <= 2^16 and r_max >= 1e-4 the test can only fire
         // when eff + log2|dz_m| >= -24: below eff = -40 it returns 0.
@@ -270,10 +275,8 @@ __device__ Sample fexp64_sample(const ap_view& v, const PertData& D, double si, 
                 if (step > 0) {
                     ++nbla;
                     if (isnan(dzx) || isnan(dzy) || isinf(dzx) || isinf(dzy)) {
-                        Sample r = nan_exit(k);
-                        r.plain = plain;
-                        r.bla = nbla;
-                        return r;
+                        leap_nan = true;
+                        break;
                     }
                     renorm64(dzx, dzy, z_exp);
                     k += step;
@@ -353,6 +356,20 @@ __device__ Sample fexp64_sample(const ap_view& v, const PertData& D, double si, 
         }
         ++k;
         ++plain;
+    }
+    if (k < ref_len) {
+        Sample r;
+        if (leap_nan) {
+            r = nan_exit(k);
+        } else {
+            const double sc = pow2_f64(z_exp - E);
+            const double zx = D.zr[k] + dzx * sc, zy = D.zi[k] + dzy * sc;
+            r = escaped(k, zx * zx + zy * zy, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
+            r.glitch = glitch;
+        }
+        r.plain = plain;
+        r.bla = nbla;
+        return r;
     }
     const int eff = z_exp - E;
     const double sfb = eff >= -60 ? pow2_f64(eff) : 0.0;
