@@ -48,15 +48,18 @@ __device__ Sample pert64_sample(const ap_view& v, const PertData& D, double si, 
         k = plain = 1;
     }
 
-    // One exit for every way out (escape at k, a non-finite leap at k, or
-    // k = ref_len); the exit is told apart after the loop, so the iterations
-    // carry nothing for the exit paths.
-    bool leap_nan = false;
     while (k < ref_len) {
         const double rx = D.zr[k], ry = D.zi[k];
         const double zx = rx + dzx, zy = ry + dzy;
         const double mag2 = zx * zx + zy * zy;
-        if (mag2 > kBailout2) break;
+        if (mag2 > kBailout2) {
+            if (strict && k > 15 && (rx * rx + ry * ry) < 4.0) glitch = 1;
+            Sample r = escaped(k, mag2, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
+            r.glitch = glitch;
+            r.plain = plain;
+            r.bla = nbla;
+            return r;
+        }
         if (strict && k > 15) {
             double r_mag2 = rx * rx + ry * ry;
             if (mag2 < 1e-6 * r_mag2 && r_mag2 > 1e-8) glitch = 1;
@@ -88,8 +91,10 @@ __device__ Sample pert64_sample(const ap_view& v, const PertData& D, double si, 
                 dzx = ndx;
                 ++nbla;
                 if (isnan(dzx) || isnan(dzy) || isinf(dzx) || isinf(dzy)) {
-                    leap_nan = true;
-                    break;
+                    Sample r = nan_exit(k);
+                    r.plain = plain;
+                    r.bla = nbla;
+                    return r;
                 }
                 k += 1 << best;
                 continue;
@@ -126,21 +131,6 @@ __device__ Sample pert64_sample(const ap_view& v, const PertData& D, double si, 
         }
         ++k;
         ++plain;
-    }
-    if (k < ref_len) {
-        Sample r;
-        if (leap_nan) {
-            r = nan_exit(k);
-        } else {
-            const double rx = D.zr[k], ry = D.zi[k];
-            const double zx = rx + dzx, zy = ry + dzy;
-            if (strict && k > 15 && (rx * rx + ry * ry) < 4.0) glitch = 1;
-            r = escaped(k, zx * zx + zy * zy, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
-            r.glitch = glitch;
-        }
-        r.plain = plain;
-        r.bla = nbla;
-        return r;
     }
     const double zx = D.zr[ref_len] + dzx, zy = D.zi[ref_len] + dzy;
     const double cx = julia ? v.julia_cx : v.ref_cx + dcx;
