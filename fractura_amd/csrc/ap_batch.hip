@@ -44,7 +44,7 @@ __global__ __launch_bounds__(kBlock) void fexp32_batch_kernel(const ap_frame* __
             d.zi = F.orbit.im;
             d.bla = bla_view(F.bla, F.orbit.len, F.view.bla_levels);
             d.zf = d.bla.zf;
-            process_pixel<EngFexp32<T>, false>(F.view, F.out, d, x, y, true, st);
+            process_pixel<EngFexp32<T>, false>(F.view, F.out, d, x, y, true, st, stats != nullptr);
         }
     }
     flush_stats(stats, st.iters, st.plain, st.bla, st.samples);

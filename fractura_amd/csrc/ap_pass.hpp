@@ -29,11 +29,13 @@ struct LaneStats {
 // One pixel.  kColour: the factor^2-subsample passes (modes 1 and 2) colour
 // every subsample here to average RGB; the centre pass (mode 0) writes the
 // maps only and RGB comes from the colourise pass, so the palette code does
-// not weigh on the registers of the iteration loop.
+// not weigh on the registers of the iteration loop.  stats: whether the
+// launch collects iteration statistics (wave-uniform; without it the
+// per-sample counting is skipped).
 template <class Eng, bool kColour>
 __device__ inline void process_pixel(const ap_view& v, const ap_outputs& o,
                                      const typename Eng::Data& d, int x, int y,
-                                     bool from_queue, LaneStats& st) {
+                                     bool from_queue, LaneStats& st, bool stats) {
     const long long idx = (long long)y * v.width + x;
     if (!from_queue) {
         if (v.rebase_pass > 0 && o.mask && o.mask[idx] == 0) return;
@@ -46,7 +48,7 @@ __device__ inline void process_pixel(const ap_view& v, const ap_outputs& o,
         const double off = ((double)f - 1.0) * 0.5;
         Sample r = Eng::sample(v, d, (double)(x * f) + off, (double)(y * f) + off);
         finish(r, inv_ref, inv_exact, v.ln_bailout_log);
-        st.add(r);
+        if (stats) st.add(r);
         if (o.counts) o.counts[idx] = r.count;
         if (o.smooth) o.smooth[idx] = r.smooth;
         if (o.iter_map) o.iter_map[idx] = r.s;
@@ -60,7 +62,7 @@ __device__ inline void process_pixel(const ap_view& v, const ap_outputs& o,
             for (int sx = 0; sx < f; ++sx) {
                 r = Eng::sample(v, d, (double)(x * f + sx), (double)(y * f + sy));
                 finish(r, inv_ref, inv_exact, v.ln_bailout_log);
-                st.add(r);
+                if (stats) st.add(r);
                 if (r.glitch) ++glitches;
                 if (r.s >= 0.0f) {
                     double cr, cg, cb;
@@ -90,7 +92,8 @@ __global__ __launch_bounds__(kBlock) void tile_pass(ap_view v, ap_outputs o,
                                                    typename Eng::Data d) {
     LaneStats st;
     int x, y;
-    if (tile_pixel(v.width, v.height, x, y)) process_pixel<Eng, kColour>(v, o, d, x, y, false, st);
+    if (tile_pixel(v.width, v.height, x, y))
+        process_pixel<Eng, kColour>(v, o, d, x, y, false, st, o.stats != nullptr);
     flush_stats(o.stats, st.iters, st.plain, st.bla, st.samples);
 }
 
@@ -107,7 +110,8 @@ __global__ __launch_bounds__(kBlock) void queue_pass(ap_view v, ap_outputs o,
         int q = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
         if (q < n) {
             int p = o.queue[q];
-            process_pixel<Eng, kColour>(v, o, d, p % v.width, p / v.width, true, st);
+            process_pixel<Eng, kColour>(v, o, d, p % v.width, p / v.width, true, st,
+                                        o.stats != nullptr);
         }
     }
     flush_stats(o.stats, st.iters, st.plain, st.bla, st.samples);
