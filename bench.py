@@ -116,6 +116,29 @@ def measured_traffic(kernel):
             "bytes": t["read_bytes"] + t["write_bytes"], "source": t["source"]}
 
 
+# MI355X: 256 CUs x 4 SIMDs; a SIMD issues one wave64 VALU instruction every
+# 4 cycles at the 2.4 GHz peak engine clock.
+SIMDS, CLOCK_HZ, CYCLES_PER_VALU = 1024, 2.4e9, 4
+
+
+def valu_issue(kernel, kernel_ms):
+    """The kernel's VALU instruction count from the committed profiler summary
+    (SQ_INSTS_VALU / SQ_WAVES, profiles/issue.json) as issue time: the time
+    the kernel would take if every SIMD issued one VALU instruction every 4
+    cycles, and that time over the kernel's measured duration."""
+    path = os.path.join(ROOT, "profiles", "issue.json")
+    try:
+        with open(path) as f:
+            c = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not c:
+        return None
+    ms = c["valu_per_wave"] * c["waves"] * CYCLES_PER_VALU / SIMDS / CLOCK_HZ * 1e3
+    return {"valu_per_wave": c["valu_per_wave"], "waves": c["waves"], "issue_ms": ms,
+            "frac": ms / kernel_ms if kernel_ms else None, "source": c["source"]}
+
+
 def roofline(bound, kernel, achieved, peak, unit, kernel_ms, **extra):
     """The roofline object of one kernel: `traffic` is its HBM bytes per
     launch (rocprof FETCH_SIZE + WRITE_SIZE, copy-calibrated; the split and
@@ -123,11 +146,14 @@ def roofline(bound, kernel, achieved, peak, unit, kernel_ms, **extra):
     the vector ALUs (no dense contraction here, so no MFMA instruction): they
     report against the compute roof ("mfma", the published FP32 157.3 /
     FP64 78.6 TFLOP/s, which the vector units share), `compute_unit` says
-    which units."""
+    which units; `valu_issue` gives the instruction-issue roof the passes
+    actually meet (the algorithmic flops are a fraction of the instructions)."""
     t = measured_traffic(kernel)
     r = {"bound": bound, "kernel": kernel, "achieved": achieved, "peak": peak, "unit": unit,
          "frac": achieved / peak if peak else None, "traffic": t["bytes"] if t else None,
          "traffic_detail": t, "kernel_ms": kernel_ms}
+    if bound == "mfma":
+        r["valu_issue"] = valu_issue(kernel, kernel_ms)
     r.update(extra)
     return r
 
