@@ -139,22 +139,31 @@ __device__ inline void uniform_leaps32(const Fexp32Data& D, float& dzx, float& d
     int ku = __builtin_amdgcn_readfirstlane(k);
     int hu = __builtin_amdgcn_readfirstlane(hi);
     if (!wave_all(k == ku && hi == hu)) return;
+    // The run keeps the exponent relative to 2^-E (eff = z_exp - E, the
+    // exponent of the actual delta): the key then needs no subtraction per
+    // leap, and the B term's exponent moves by E in scalar registers.  Every
+    // exponent of the leap is shifted by the same E, so the shifts of the
+    // mantissas, and with them the results, are those of the per-lane leap.
+    int eff = z_exp - E;
+    const int eff_deep = z_deep - E;
     int steps = 0;
     for (;;) {
-        const Coef32 c = scalar_entry(D.bla.c32, D.bla.at(ku, hu));
+        Coef32 c = scalar_entry(D.bla.c32, D.bla.at(ku, hu));
         const float q = fmaf(dzx, dzx, dzy * dzy);
-        if (!wave_all(dz_key(q, z_exp - E) < c.key)) break;
+        if (!wave_all(dz_key(q, eff) < c.key)) break;
+        c.be -= E;
         float rx, ry;
         int ne;
-        leap32_terms<kNoDc>(c, q, dzx, dzy, z_exp, dpx, dpy, rx, ry, ne);
+        leap32_terms<kNoDc>(c, q, dzx, dzy, eff, dpx, dpy, rx, ry, ne);
         dzx = rx;
         dzy = ry;
-        z_exp = ne;
+        eff = ne;
         ku += 1 << hu;
         hu = c.next_hi;
         ++steps;
-        if (hu < 0 || !wave_all(z_exp < z_deep)) break;
+        if (hu < 0 || !wave_all(eff < eff_deep)) break;
     }
+    z_exp = eff + E;
     k = ku;
     hi = hu;
     nbla += steps;
