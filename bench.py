@@ -88,17 +88,44 @@ class Span:
 
 def time_graph(torch, graph, warmup, steps):
     """Wall time per replay of a captured frame (barrier + synchronize on both
-    sides of the timed loop) and the mean event time of one replay."""
+    sides of the timed loop, nothing else in it) and, from a second loop, the
+    mean event time of one replay."""
     for _ in range(warmup):
         graph.replay()
     torch.cuda.synchronize()
-    timer = Timer(torch)
     t0 = time.perf_counter()
+    for _ in range(steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    timer = Timer(torch)
     for _ in range(steps):
         with Span(timer):
             graph.replay()
+    return wall, timer.mean_ms()
+
+
+def time_in_flight(torch, graphs, warmup, steps):
+    """Wall time per frame with len(graphs) frames in flight: step i replays
+    graph i mod n on stream i mod n (each graph has its own table and output
+    buffers, as the reference's stream slots do), so one frame's latency-bound
+    table build and colouring overlap another's pixel pass.  Streams keep the
+    default priority; barrier + synchronize on both sides of the timed loop."""
+    n = len(graphs)
+    cur = torch.cuda.current_stream()
+    streams = [torch.cuda.Stream() for _ in range(n)]
+    for st in streams:
+        st.wait_stream(cur)
+    for i in range(warmup * n):
+        with torch.cuda.stream(streams[i % n]):
+            graphs[i % n].replay()
     torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / steps, timer.mean_ms()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        with torch.cuda.stream(streams[i % n]):
+            graphs[i % n].replay()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
 
 
 def measured_traffic(kernel):
@@ -183,6 +210,11 @@ class Timer:
         return float(np.mean([a.elapsed_time(b) for a, b in self.pairs]))
 
 
+# Frames in flight in the config-3 timing (stream slots, each with its own
+# table and outputs).
+IN_FLIGHT = 2
+
+
 def run_deep(args, torch):
     from fractura_amd import _abi, engine
 
@@ -202,24 +234,29 @@ def run_deep(args, torch):
     assert plan32.engine == "fexp32" and plan64.engine == "fexp64"
     dev = engine.get_device(0)
     dev.set_stream(None)
-    orbit = engine._Orbit(dev, plan32, 0, forms=_abi.AP_BLA_FORM_ALL)
+    # One table and one set of outputs per frame in flight (a stream slot).
+    orbits = [engine._Orbit(dev, plan32, 0, tag=f"bench{i}", forms=_abi.AP_BLA_FORM_ALL)
+              for i in range(IN_FLIGHT)]
     lut = engine.get_gpu_cmap_lut("inferno", 2048, False, dev.torch_device)
     n = W * H
     outs = {}
     for name in ("fexp32", "fexp64"):
-        outs[name] = dict(
+        outs[name] = [dict(
             counts=torch.empty((H, W), dtype=torch.int32, device=dev.torch_device),
             smooth=torch.empty((H, W), dtype=torch.float64, device=dev.torch_device),
             iter_map=torch.empty((H, W), dtype=torch.float32, device=dev.torch_device),
             glitch=torch.empty((H, W), dtype=torch.uint8, device=dev.torch_device),
             rgb=torch.empty((H, W, 3), dtype=torch.uint8, device=dev.torch_device),
             stats=torch.zeros(4, dtype=torch.int64, device=dev.torch_device))
+            for _ in range(IN_FLIGHT)]
 
-    def step(name, plan, timers=None, with_stats=False):
-        """BLA build, pixel pass, colouring: each bracketed by its own HIP
-        events (on the stream the library launches on) when timed."""
+    def step(name, plan, slot=0, timers=None, with_stats=False):
+        """BLA build, pixel pass, colouring of one frame on slot `slot`'s
+        buffers: each part bracketed by its own HIP events (on the stream the
+        library launches on) when timed."""
         v = plan.view.copy()
-        o = outs[name]
+        o = outs[name][slot]
+        orbit = orbits[slot]
         ao = engine._outs(o["counts"], o["smooth"], o["iter_map"], o["glitch"],
                           stats=o["stats"] if with_stats else None)
         t = timers or {}
@@ -235,10 +272,10 @@ def run_deep(args, torch):
 
     results = {}
     for name, plan in (("fexp32", plan32), ("fexp64", plan64)):
-        outs[name]["stats"].zero_()
+        outs[name][0]["stats"].zero_()
         step(name, plan, with_stats=True)
         torch.cuda.synchronize()
-        st = outs[name]["stats"].cpu().numpy().view(np.uint64)
+        st = outs[name][0]["stats"].cpu().numpy().view(np.uint64)
         iters, plain, bla = int(st[0]), int(st[1]), int(st[2])
         for _ in range(args.warmup):
             step(name, plan)
@@ -248,25 +285,34 @@ def run_deep(args, torch):
         timers = {k: Timer(torch) for k in ("build", "pass", "colour")}
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            step(name, plan, timers)
+            step(name, plan, timers=timers)
         torch.cuda.synchronize()
         eager_wall = (time.perf_counter() - t0) / args.steps
-        # The frame as one HIP graph (build, pass, colouring): the timed steps.
-        wall, graph_ms = time_graph(torch, engine.capture_graph(dev, lambda: step(name, plan)),
-                                    args.warmup, args.steps)
+        # The frame as one HIP graph per slot (build, pass, colouring): one
+        # frame at a time on one stream, then IN_FLIGHT frames in flight (the
+        # timed steps of `value`).
+        graphs = [engine.capture_graph(dev, lambda i=i: step(name, plan, slot=i))
+                  for i in range(IN_FLIGHT)]
+        wall1, graph_ms = time_graph(torch, graphs[0], args.warmup, args.steps)
+        wall = time_in_flight(torch, graphs, args.warmup, args.steps)
+        for i in range(1, IN_FLIGHT):  # the slots rendered the same frame
+            for k in ("counts", "smooth", "rgb"):
+                assert torch.equal(outs[name][0][k], outs[name][i][k]), (name, i, k)
         kern_ms = timers["pass"].mean_ms()
         flops = plain * FLOPS_FEXP_PLAIN + bla * FLOPS_BLA_STEP
         results[name] = dict(iters=iters, plain=plain, bla=bla, wall_ms=wall * 1e3,
-                             graph_ms=graph_ms, eager_wall_ms=eager_wall * 1e3,
-                             kernel_ms=kern_ms, build_ms=timers["build"].mean_ms(),
+                             single_wall_ms=wall1 * 1e3, graph_ms=graph_ms,
+                             eager_wall_ms=eager_wall * 1e3, kernel_ms=kern_ms,
+                             build_ms=timers["build"].mean_ms(),
                              colour_ms=timers["colour"].mean_ms(), giters=iters / wall / 1e9,
+                             single_giters=iters / wall1 / 1e9,
                              tflops=flops / (kern_ms * 1e-3) / 1e12 if kern_ms > 0 else 0.0)
         log(f"{name}: {results[name]}")
 
-    c32 = outs["fexp32"]["counts"].cpu().numpy()
-    c64 = outs["fexp64"]["counts"].cpu().numpy()
-    s32 = outs["fexp32"]["smooth"].cpu().numpy()
-    s64 = outs["fexp64"]["smooth"].cpu().numpy()
+    c32 = outs["fexp32"][0]["counts"].cpu().numpy()
+    c64 = outs["fexp64"][0]["counts"].cpu().numpy()
+    s32 = outs["fexp32"][0]["smooth"].cpu().numpy()
+    s64 = outs["fexp64"][0]["smooth"].cpu().numpy()
     agree = float(np.mean(c32 == c64))
     esc = (s64 >= 0) & (s32 >= 0)
     smooth_err = float(np.max(np.abs(s32[esc] - s64[esc]), initial=0.0))
@@ -291,6 +337,9 @@ def run_deep(args, torch):
                    "size": [W, H], "max_iter": max_iter, "bla_levels": 16, "bla_tol": 1e-6,
                    "glitch": "Pauldelbrot 1e-3", "ref_len": plan32.view.ref_len,
                    "e_scale": plan32.view.e_scale},
+        "frames_in_flight": IN_FLIGHT,
+        "one_frame_at_a_time": {"value": r32["single_giters"], "ms_per_step": r32["single_wall_ms"],
+                                "fp64_mantissa_value": r64["single_giters"]},
         "fp64_mantissa_value": r64["giters"],
         "fp32_vs_fp64": r32["giters"] / r64["giters"] if r64["giters"] else None,
         "count_agreement_fp32_vs_fp64": agree,
@@ -306,7 +355,8 @@ def run_deep(args, torch):
                                   compute_unit="VALU fp64"),
         "step_ms": {e: {"bla_build": r["build_ms"], "pass": r["kernel_ms"],
                         "colour": r["colour_ms"], "graph_replay": r["graph_ms"],
-                        "wall": r["wall_ms"], "eager_wall": r["eager_wall_ms"]}
+                        "wall": r["wall_ms"], "wall_one_at_a_time": r["single_wall_ms"],
+                        "eager_wall": r["eager_wall_ms"]}
                     for e, r in (("fexp32", r32), ("fexp64", r64))},
         "colour_pass": colour_roofline(n, r32["colour_ms"]),
         "cpu_baseline": cpu,
