@@ -568,9 +568,9 @@ Res fexp64_sample(const ap_view& v, const Orbit& o, double si, double sj) {
 }
 
 // ------------------------------------------------ FloatExp FP32 -------------
-// CPU restatement of this build's FP32-mantissa FloatExp loop (it has no
+// CPU restatement of the device's FP32-mantissa FloatExp loop (it has no
 // reference counterpart; the reference FloatExp mantissa is FP64).  It pins
-// the device kernel's arithmetic; its accuracy is judged against the FP64
+// the device kernel's arithmetic; its accuracy is measured against the FP64
 // loop above and against mpmath.  The FP32 BLA forms are derived from the
 // reference-layout table exactly as the device derives them.
 struct Coef32 {
@@ -960,7 +960,7 @@ void put_rgb(void* rgb, int hdr, size_t idx, double r, double g, double b) {
 // Signed-magnitude numbers of L base-2^32 limbs: limb 0 is the integer part,
 // limbs 1..L-1 the fraction, most significant first
 // (math_core/cuda_kernels.py:2237-2364).  The reference's GPU tier fixes
-// L = 34; the same arithmetic at larger L is this build's replacement for
+// L = 34; the same arithmetic at larger L replaces
 // the reference's arbitrary-precision CPU tier.
 const int kMaxLimbs = AP_MAX_LIMBS;
 
@@ -1387,7 +1387,7 @@ int or_render_fexp64(const ap_view* v, const double* zr, const double* zi, const
                   [&](double si, double sj) { return fexp64_sample(V, ob, si, sj); }, iters);
 }
 
-// FP32-mantissa FloatExp (this build's design) over the same orbit and a
+// FP32-mantissa FloatExp (the device's design) over the same orbit and a
 // reference-layout table.
 int or_render_fexp32(const ap_view* v, const double* zr, const double* zi, const double* bla,
                      int levels, const ap_outputs* o, int threads, long long* iters) {
