@@ -74,3 +74,48 @@ def test_graph_replays_match_eager(name, w, max_iter):
         # Eager library work between replays.
         fn()
         torch.cuda.synchronize()
+
+
+def test_two_frames_in_flight_on_two_streams():
+    """Two slots (own table and outputs) replayed alternately on two streams,
+    as the benchmark times them: every replay of either slot gives the eager
+    frame, byte for byte."""
+    import torch
+
+    W, H = 160, 90
+    plan = engine.plan_view(W, H, 4000, Decimal("0"), Decimal("1"), Decimal("1e-320"),
+                            bbsa_order=32, bbsa_tol=1e-6, glitch_mode="Pauldelbrot",
+                            fexp_mantissa=32)
+    dev = engine.get_device(0)
+    dev.set_stream(None)
+    lut = engine.get_gpu_cmap_lut("inferno", 2048, False, dev.torch_device)
+    slots = []
+    for i in range(2):
+        orbit = engine._Orbit(dev, plan, 0, tag=f"inflight{i}")
+        bufs = dict(counts=torch.zeros((H, W), dtype=torch.int32, device=dev.torch_device),
+                    smooth=torch.zeros((H, W), dtype=torch.float64, device=dev.torch_device),
+                    iter_map=torch.zeros((H, W), dtype=torch.float32, device=dev.torch_device),
+                    glitch=torch.zeros((H, W), dtype=torch.uint8, device=dev.torch_device),
+                    rgb=torch.zeros((H, W, 3), dtype=torch.uint8, device=dev.torch_device))
+        slots.append((_frame_fn(dev, "fexp32", plan, orbit, bufs, lut), bufs))
+    slots[0][0]()
+    torch.cuda.synchronize()
+    ref = {k: to_np(t).copy() for k, t in slots[0][1].items()}
+    assert np.mean(ref["counts"] < 4000) > 0.3
+    graphs = [engine.capture_graph(dev, fn) for fn, _ in slots]
+    streams = [torch.cuda.Stream(device=dev.torch_device) for _ in range(2)]
+    for rep in range(3):
+        for _, bufs in slots:
+            for t in bufs.values():
+                t.zero_()
+        torch.cuda.synchronize()
+        for st in streams:
+            st.wait_stream(torch.cuda.current_stream())
+        for i in range(6):
+            with torch.cuda.stream(streams[i % 2]):
+                graphs[i % 2].replay()
+        torch.cuda.synchronize()
+        for j, (_, bufs) in enumerate(slots):
+            for k, t in bufs.items():
+                got = to_np(t)
+                assert np.array_equal(got.view(np.uint8), ref[k].view(np.uint8)), (rep, j, k)
