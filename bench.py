@@ -210,11 +210,6 @@ class Timer:
         return float(np.mean([a.elapsed_time(b) for a, b in self.pairs]))
 
 
-# Frames in flight in the config-3 timing (stream slots, each with its own
-# table and outputs).
-IN_FLIGHT = 2
-
-
 def run_deep(args, torch):
     from fractura_amd import _abi, engine
 
@@ -234,6 +229,7 @@ def run_deep(args, torch):
     assert plan32.engine == "fexp32" and plan64.engine == "fexp64"
     dev = engine.get_device(0)
     dev.set_stream(None)
+    IN_FLIGHT = max(1, args.in_flight)
     # One table and one set of outputs per frame in flight (a stream slot).
     orbits = [engine._Orbit(dev, plan32, 0, tag=f"bench{i}", forms=_abi.AP_BLA_FORM_ALL)
               for i in range(IN_FLIGHT)]
@@ -521,13 +517,15 @@ def run_fp64(args, torch):
 
 def prepare_flythrough(args):
     """Host side of the config-5 fly-through, before the GPU is touched: the
-    64 frames' reference orbits, computed in a forked process pool."""
+    64 frames' reference orbits, computed in a forked process pool (or in this
+    process with --orbit-processes 1, for runs under a profiler, whose
+    preloaded library the forked workers would inherit)."""
     from fractura_amd import batch
 
     cfg5 = load_configs()["config5"]
     specs = batch.flythrough_specs(cfg5)
     t0 = time.time()
-    plans = batch.plan_frames(specs)
+    plans = batch.plan_frames(specs, processes=args.orbit_processes or None)
     log(f"flythrough: {len(plans)} frame plans in {time.time() - t0:.1f}s "
         f"(ref_len {min(p.view.ref_len for p in plans)}..{max(p.view.ref_len for p in plans)})")
     return cfg5, plans
@@ -541,11 +539,16 @@ def run_flythrough(args, torch, prepared):
     from fractura_amd import batch, engine
 
     cfg5, plans = prepared
-    b = batch.FrameBatch(plans)
-    lut = engine.get_gpu_cmap_lut("inferno", 2048, False, b.dev.torch_device)
-    tuned = batch.autotune(b, reps=3)
-    log(f"autotune: tile {tuned['tile']} blocks {tuned['blocks']} "
-        f"({tuned['ms']:.3f} ms; {len(tuned['candidates'])} candidates)")
+    lut = engine.get_gpu_cmap_lut("inferno", 2048, False, torch.device("cuda", 0))
+    if args.tile:
+        # A fixed launch shape (profiling runs): no autotuning.
+        b = batch.FrameBatch(plans, tile=tuple(args.tile), blocks=args.blocks)
+        tuned = {"tile": list(args.tile), "blocks": args.blocks, "ms": None, "candidates": []}
+    else:
+        b = batch.FrameBatch(plans)
+        tuned = batch.autotune(b, reps=3)
+        log(f"autotune: tile {tuned['tile']} blocks {tuned['blocks']} "
+            f"({tuned['ms']:.3f} ms; {len(tuned['candidates'])} candidates)")
     b.step(lut)
     b.stats.zero_()
     b.render(stats=True)
@@ -657,6 +660,14 @@ def main():
     ap.add_argument("--size", type=int, nargs=2, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--orbit-processes", type=int, default=0,
+                    help="fly-through: host orbit processes (0 = one per core, up to 16)")
+    ap.add_argument("--in-flight", type=int, default=2,
+                    help="config 3: frames in flight (stream slots) in the timed loop")
+    ap.add_argument("--tile", type=int, nargs=2, default=None,
+                    help="fly-through: fixed tile shape instead of autotuning")
+    ap.add_argument("--blocks", type=int, default=0,
+                    help="fly-through with --tile: workgroups (0 = fill by occupancy)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1",
