@@ -291,9 +291,11 @@ def run_deep(args, torch):
                   for i in range(IN_FLIGHT)]
         wall1, graph_ms = time_graph(torch, graphs[0], args.warmup, args.steps)
         wall = time_in_flight(torch, graphs, args.warmup, args.steps)
-        for i in range(1, IN_FLIGHT):  # the slots rendered the same frame
-            for k in ("counts", "smooth", "rgb"):
-                assert torch.equal(outs[name][0][k], outs[name][i][k]), (name, i, k)
+        # The slots rendered the same frame: their outputs must agree.
+        slots_equal = all(torch.equal(outs[name][0][k], outs[name][i][k])
+                          for i in range(1, IN_FLIGHT) for k in ("counts", "smooth", "rgb"))
+        if not slots_equal:
+            log(f"{name}: the {IN_FLIGHT} slots' outputs differ")
         kern_ms = timers["pass"].mean_ms()
         flops = plain * FLOPS_FEXP_PLAIN + bla * FLOPS_BLA_STEP
         results[name] = dict(iters=iters, plain=plain, bla=bla, wall_ms=wall * 1e3,
@@ -301,7 +303,7 @@ def run_deep(args, torch):
                              eager_wall_ms=eager_wall * 1e3, kernel_ms=kern_ms,
                              build_ms=timers["build"].mean_ms(),
                              colour_ms=timers["colour"].mean_ms(), giters=iters / wall / 1e9,
-                             single_giters=iters / wall1 / 1e9,
+                             single_giters=iters / wall1 / 1e9, slots_equal=slots_equal,
                              tflops=flops / (kern_ms * 1e-3) / 1e12 if kern_ms > 0 else 0.0)
         log(f"{name}: {results[name]}")
 
@@ -334,6 +336,7 @@ def run_deep(args, torch):
                    "glitch": "Pauldelbrot 1e-3", "ref_len": plan32.view.ref_len,
                    "e_scale": plan32.view.e_scale},
         "frames_in_flight": IN_FLIGHT,
+        "slots_equal": r32["slots_equal"] and r64["slots_equal"],
         "one_frame_at_a_time": {"value": r32["single_giters"], "ms_per_step": r32["single_wall_ms"],
                                 "fp64_mantissa_value": r64["single_giters"]},
         "fp64_mantissa_value": r64["giters"],
