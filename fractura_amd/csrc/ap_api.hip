@@ -95,6 +95,8 @@ int check_orbit(ap_ctx* c, const ap_view* v, const ap_orbit* orb, const void* bl
     if (!orb || !orb->re || !orb->im) return fail(c, AP_ERR_ARG, "null orbit");
     if (orb->len < 0 || orb->len != v->ref_len)
         return fail(c, AP_ERR_ARG, "orbit length %d does not match view ref_len %d", orb ? orb->len : -1, v->ref_len);
+    // The passes address the orbit arrays with 32-bit byte offsets.
+    if (orb->len >= (1 << 29)) return fail(c, AP_ERR_ARG, "orbit length %d exceeds 2^29 - 1", orb->len);
     if (v->bla_levels > 1) {
         if (!bla) return fail(c, AP_ERR_ARG, "bla_levels=%d but no table", v->bla_levels);
         if (v->bla_levels > 31) return fail(c, AP_ERR_ARG, "bla_levels=%d > 31", v->bla_levels);

@@ -286,7 +286,7 @@ __device__ Sample fexp32_tail_loop(const ap_view& v, const Fexp32Data& D, const 
     // escape is re-tested after it from the same operands, so no value of the
     // escape path has to be carried through the iterations.
     while (k < ref_len) {
-        const double rx = D.zr[k], ry = D.zi[k];
+        const double rx = entry(D.zr, (unsigned)k), ry = entry(D.zi, (unsigned)k);
         const double zx = rx + dzx, zy = ry + dzy;
         const double mag2 = zx * zx + zy * zy;
         if (mag2 > kBailout2) break;
@@ -299,7 +299,7 @@ __device__ Sample fexp32_tail_loop(const ap_view& v, const Fexp32Data& D, const 
         ++plain;
     }
     if (k < ref_len) {
-        const double zx = D.zr[k] + dzx, zy = D.zi[k] + dzy;
+        const double zx = entry(D.zr, (unsigned)k) + dzx, zy = entry(D.zi, (unsigned)k) + dzy;
         Sample r = escaped(k, zx * zx + zy * zy, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
         r.glitch = glitch;
         r.plain = plain;
@@ -368,7 +368,7 @@ __device__ Sample fexp32_sample(const ap_view& v, const Fexp32Data& D, double si
             float m = fmaxf(fabsf(dzx), fabsf(dzy));
             if (m > 0.0f && eff + ilogbf(m) >= kTail32) break;
         }
-        const float2 Z = D.zf[k];
+        const float2 Z = entry(D.zf, (unsigned)k);
         if (strict && k > 15 && eff >= -40) {
             // |Z + dz|^2 < 1e-6 |Z|^2 in actual units, for |Z| >= 1e-4.
             float rmax = fmaxf(fabsf(Z.x), fabsf(Z.y));

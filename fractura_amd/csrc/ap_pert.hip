@@ -49,7 +49,7 @@ __device__ Sample pert64_sample(const ap_view& v, const PertData& D, double si, 
     }
 
     while (k < ref_len) {
-        const double rx = D.zr[k], ry = D.zi[k];
+        const double rx = entry(D.zr, (unsigned)k), ry = entry(D.zi, (unsigned)k);
         const double zx = rx + dzx, zy = ry + dzy;
         const double mag2 = zx * zx + zy * zy;
         if (mag2 > kBailout2) {
@@ -246,7 +246,7 @@ __device__ Sample fexp64_sample(const ap_view& v, const PertData& D, double si, 
             }
             if (leap_nan || k >= ref_len) break;
         }
-        const double rx = D.zr[k], ry = D.zi[k];
+        const double rx = entry(D.zr, (unsigned)k), ry = entry(D.zi, (unsigned)k);
         const int eff = z_exp - E;
         if (eff >= -50) {
             double sc = pow2_f64(eff);
@@ -353,7 +353,7 @@ __device__ Sample fexp64_sample(const ap_view& v, const PertData& D, double si, 
             r = nan_exit(k);
         } else {
             const double sc = pow2_f64(z_exp - E);
-            const double zx = D.zr[k] + dzx * sc, zy = D.zi[k] + dzy * sc;
+            const double zx = entry(D.zr, (unsigned)k) + dzx * sc, zy = entry(D.zi, (unsigned)k) + dzy * sc;
             r = escaped(k, zx * zx + zy * zy, pc.inv_ref, pc.inv_exact, v.ln_bailout_log);
             r.glitch = glitch;
         }

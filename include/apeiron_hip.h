@@ -136,7 +136,7 @@ typedef struct ap_outputs {
 typedef struct ap_orbit {
     const double *re;
     const double *im;
-    int32_t len;          /* ref_len; arrays hold len+1 entries              */
+    int32_t len;          /* ref_len < 2^29; arrays hold len+1 entries       */
     int32_t reserved0;
 } ap_orbit;
 

@@ -156,6 +156,14 @@ def test_oversize_frame_is_rejected_before_launch():
     st = dev.lib.ap_render_fp64(dev.handle, C.byref(v), C.byref(o))
     assert st == _abi.AP_ERR_ARG
     assert b"exceeds 2^31" in dev.lib.ap_last_error(dev.handle)
+    # An orbit too long for the passes' 32-bit offsets (pointers never read).
+    v2 = _abi.ApView()
+    v2.width, v2.height, v2.factor, v2.max_iter = 4, 4, 1, 10
+    v2.ref_len = 1 << 29
+    orb = _abi.ApOrbit(16, 16, 1 << 29, 0)
+    st = dev.lib.ap_render_pert64(dev.handle, C.byref(v2), C.byref(orb), None, C.byref(o))
+    assert st == _abi.AP_ERR_ARG
+    assert b"exceeds 2^29" in dev.lib.ap_last_error(dev.handle)
     # Limb evaluator: more limbs than it holds.
     st = dev.lib.ap_render_limbs(dev.handle, C.byref(v), _abi.AP_LIMBS_HIGHPREC,
                                  _abi.AP_MAX_LIMBS + 1, None, 0, None, None, None, None,
