@@ -32,6 +32,7 @@ AP_BLA_FORM_ALL = 3
 AP_LIMBS_GPU34 = 0
 AP_LIMBS_HIGHPREC = 1
 AP_MAX_LIMBS = 256
+AP_MAX_ORBIT_LIMBS = 4096
 
 
 class ApView(C.Structure):
@@ -118,6 +119,7 @@ SIGNATURES = {
     "ap_render_dd": (C.c_int, [_P, _VIEW, _OUT]),
     "ap_render_limbs": (C.c_int, [_P, _VIEW, _I, _I, _P, _I, _P, _P, _P, _P, _OUT]),
     "ap_bbsa_coeffs": (C.c_int, [_P, _P, _I, _D, _I, _D, _I, _I, _D, _D, _I, _P]),
+    "ap_reference_orbit": (C.c_int, [_I, _I, _I, _I, _P, _P, _I, _P, _P, _P]),
     "ap_render_pert64": (C.c_int, [_P, _VIEW, _ORB, _P, _OUT]),
     "ap_render_fexp64": (C.c_int, [_P, _VIEW, _ORB, _P, _OUT]),
     "ap_render_fexp32": (C.c_int, [_P, _VIEW, _ORB, _P, _OUT]),

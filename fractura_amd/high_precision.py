@@ -2,9 +2,9 @@
 
 The reference computes Z_k in MPFR (gmpy2) or Decimal and stores float(Z_k)
 (math_core/high_precision.py:706-875).  Here the orbit runs in binary fixed
-point on Python integers: Z = X / 2^P with P = precision bits + 64 guard
-bits, products truncated by a right shift.  float(Z) is the correctly
-rounded int/int division.  The algorithm (storage until |float Z|^2 > 65536,
+point (the library's host routine ap_reference_orbit, C++): Z = X / 2^P with
+P = precision bits + 64 guard bits, products truncated by a right shift.
+float(Z) is correctly rounded.  The algorithm (storage until |float Z|^2 > 65536,
 period detection on exact equality at doubling checkpoints for types 0/2,
 Z_ref_len stored after the loop) follows the reference; only the rounding of
 the big-number arithmetic differs, far below the stored double's precision.
@@ -48,83 +48,57 @@ def _to_float(X, P):
     return X / (1 << P)
 
 
+def _limbs_of(values):
+    """Six fixed-point integers as (6, limbs) little-endian uint64 magnitudes
+    and int32 signs."""
+    nbits = max(max(abs(v).bit_length() for v in values), 1)
+    limbs = nbits // 64 + 1
+    mags = np.zeros((len(values), limbs), dtype=np.uint64)
+    signs = np.zeros(len(values), dtype=np.int32)
+    for i, v in enumerate(values):
+        signs[i] = 1 if v < 0 else 0
+        mags[i] = np.frombuffer(abs(v).to_bytes(limbs * 8, "little"), dtype=np.uint64)
+    return mags, signs, limbs
+
+
 def compute_single_highprec_orbit(cand_cx_dec, cand_cy_dec, max_iter, fractal_type=0,
                                   julia_cx=0.0, julia_cy=0.0, prec_bits=256,
                                   gen_n=3, gen_kr=0.25, gen_ki=1.0):
     """Reference orbit float64 arrays (re, im) of length max_iter+1 and its
-    escape index ref_len (math_core/high_precision.py:706-875)."""
+    escape index ref_len (math_core/high_precision.py:706-875), iterated in
+    binary fixed point with P = prec_bits + 64 guard bits by the library's
+    host routine ap_reference_orbit."""
+    import ctypes as C
+
+    from . import _abi
+
     P = int(prec_bits) + GUARD_BITS
-    ref_re = np.zeros(max_iter + 1, dtype=np.float64)
-    ref_im = np.zeros(max_iter + 1, dtype=np.float64)
+    max_iter = int(max_iter)
+    fractal_type = int(fractal_type)
     if fractal_type in (2, 3):
         X, Y = _fixed(cand_cx_dec, P), _fixed(cand_cy_dec, P)
         CX, CY = _fixed(julia_cx, P), _fixed(julia_cy, P)
     else:
         X, Y = 0, 0
         CX, CY = _fixed(cand_cx_dec, P), _fixed(cand_cy_dec, P)
-    one = 1 << P
-    ref_len = max_iter
-    sh = P
-    if fractal_type in (0, 2):
-        check_step = 8
-        saved_k = 0
-        sX, sY = X, Y
-        sfx, sfy = 0.0, 0.0
-        for k in range(max_iter):
-            rx, ry = X / one, Y / one
-            ref_re[k] = rx
-            ref_im[k] = ry
-            if rx * rx + ry * ry > 65536.0:
-                ref_len = k
-                break
-            if k == check_step:
-                check_step <<= 1
-                saved_k = k
-                sX, sY, sfx, sfy = X, Y, rx, ry
-            elif k > saved_k and rx == sfx and ry == sfy and X == sX and Y == sY:
-                period = k - saved_k
-                for j in range(k, max_iter + 1):
-                    ref_re[j] = ref_re[j - period]
-                    ref_im[j] = ref_im[j - period]
-                return ref_re, ref_im, max_iter
-            X2 = (X * X) >> sh
-            Y2 = (Y * Y) >> sh
-            Y = ((X * Y) >> (sh - 1)) + CY
-            X = X2 - Y2 + CX
-        ref_re[ref_len] = X / one
-        ref_im[ref_len] = Y / one
-    elif fractal_type in (1, 3):
-        for k in range(max_iter):
-            rx, ry = X / one, Y / one
-            ref_re[k] = rx
-            ref_im[k] = ry
-            if rx * rx + ry * ry > 65536.0:
-                ref_len = k
-                break
-            X2 = (X * X) >> sh
-            Y2 = (Y * Y) >> sh
-            Y = ((abs(X) * abs(Y)) >> (sh - 1)) - CY
-            X = X2 - Y2 + CX
-        ref_re[ref_len] = X / one
-        ref_im[ref_len] = Y / one
-    else:
-        KR, KI = _fixed(gen_kr, P), _fixed(gen_ki, P)
-        for k in range(max_iter):
-            rx, ry = X / one, Y / one
-            ref_re[k] = rx
-            ref_im[k] = ry
-            if rx * rx + ry * ry > 65536.0:
-                ref_len = k
-                break
-            pr, pi = X, Y
-            for _ in range(gen_n - 1):
-                pr, pi = (pr * X - pi * Y) >> sh, (pr * Y + pi * X) >> sh
-            kzr = (KR * X - KI * Y) >> sh
-            kzi = (KR * Y + KI * X) >> sh
-            X, Y = pr + kzr + CX, pi + kzi + CY
-        ref_re[ref_len] = X / one
-        ref_im[ref_len] = Y / one
-    return ref_re, ref_im, ref_len
+    KR, KI = (_fixed(gen_kr, P), _fixed(gen_ki, P)) if fractal_type == 4 else (0, 0)
+    mags, signs, limbs = _limbs_of([X, Y, CX, CY, KR, KI])
+    if limbs > _abi.AP_MAX_ORBIT_LIMBS:
+        raise ValueError(f"orbit precision of {limbs * 64} bits exceeds the library's limit")
+    ref_re = np.zeros(max_iter + 1, dtype=np.float64)
+    ref_im = np.zeros(max_iter + 1, dtype=np.float64)
+    rl = C.c_int32(0)
+    lib = _abi.load()
+    dp = C.POINTER(C.c_double)
+    st = lib.ap_reference_orbit(fractal_type, max_iter, P, limbs,
+                                mags.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                signs.ctypes.data_as(C.POINTER(C.c_int32)),
+                                int(gen_n) if fractal_type == 4 else 2,
+                                ref_re.ctypes.data_as(dp), ref_im.ctypes.data_as(dp), C.byref(rl))
+    if st != _abi.AP_OK:
+        raise ValueError(f"ap_reference_orbit rejected type={fractal_type} max_iter={max_iter} "
+                         f"P={P} gen_n={gen_n} (status {st})")
+    return ref_re, ref_im, int(rl.value)
 
 
 def compute_highprec_escape_scalar(cx_dec, cy_dec, max_iter, fractal_type=0, julia_cx=0.0,

@@ -240,6 +240,23 @@ int ap_render_limbs(ap_ctx *ctx, const ap_view *view, int32_t semantics, int32_t
                     const int32_t *signs, const uint32_t *params,
                     const int32_t *param_signs, const ap_outputs *out);
 
+/* Reference orbit of a view in binary fixed point (host code, no device):
+ * replaces compute_single_highprec_orbit (math_core/high_precision.py:
+ * 706-875), which iterates in MPFR/Decimal and stores float(Z_k).  Z = X/2^P
+ * with P = frac_bits; init holds six fixed-point integers of `limbs` 64-bit
+ * little-endian magnitude limbs each (X0, Y0, CX, CY, KR, KI: the start, the
+ * additive constant and the type-4 k), signs[6] their signs (non-zero =
+ * negative).  Products are truncated towards minus infinity by a shift of
+ * P bits; float(Z) is correctly rounded.  Writes re/im[0..max_iter] (zeros
+ * past ref_len + 1, or the periodic continuation when types 0/2 find a
+ * period, ref_len = max_iter then) and *ref_len, the first k with
+ * |float Z_k|^2 > 65536 (max_iter if none).  limbs <= AP_MAX_ORBIT_LIMBS;
+ * type 4 takes 2 <= gen_n <= 16. */
+#define AP_MAX_ORBIT_LIMBS 4096
+int ap_reference_orbit(int32_t fractal_type, int32_t max_iter, int32_t frac_bits, int32_t limbs,
+                       const uint64_t *init, const int32_t *signs, int32_t gen_n, double *re,
+                       double *im, int32_t *ref_len);
+
 /* ---- series approximation ----------------------------------------------------
  * Replaces compute_bbsa_coeffs / _jit_compute_bbsa_coeffs
  * (math_core/high_precision.py:51-536) and compute_bbsa_coeffs_cuda

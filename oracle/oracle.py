@@ -191,3 +191,70 @@ def limb_op(op, a, b, a_sign=0, b_sign=0):
     out = np.zeros_like(a)
     r = load().or_limb_op(op, len(a), a_sign, _p(a), b_sign, _p(b), _p(out))
     return r, out
+
+
+def highprec_orbit(X, Y, CX, CY, P, max_iter, fractal_type=0, gen_n=3, KR=0, KI=0):
+    """Reference orbit in binary fixed point on Python integers, the
+    restatement that pins the library's ap_reference_orbit: the storage and
+    escape rule, the period detection of types 0/2 at doubling checkpoints and
+    the store of Z_ref_len follow compute_single_highprec_orbit
+    (math_core/high_precision.py:706-875, period check :749-761), with
+    Z = X / 2^P, every product truncated by Python's floor shift and float(Z)
+    the correctly rounded int / int division.  X, Y: the start (the pixel for
+    Julia types, else 0); CX, CY: the additive constant; KR, KI: the type-4 k;
+    all fixed-point integers with P fraction bits."""
+    ref_re = np.zeros(max_iter + 1, dtype=np.float64)
+    ref_im = np.zeros(max_iter + 1, dtype=np.float64)
+    one = 1 << P
+    sh = P
+    ref_len = max_iter
+    if fractal_type in (0, 2):
+        check_step, saved_k = 8, 0
+        sX, sY, sfx, sfy = X, Y, 0.0, 0.0
+        for k in range(max_iter):
+            rx, ry = X / one, Y / one
+            ref_re[k], ref_im[k] = rx, ry
+            if rx * rx + ry * ry > 65536.0:
+                ref_len = k
+                break
+            if k == check_step:
+                check_step <<= 1
+                saved_k = k
+                sX, sY, sfx, sfy = X, Y, rx, ry
+            elif k > saved_k and rx == sfx and ry == sfy and X == sX and Y == sY:
+                period = k - saved_k
+                for j in range(k, max_iter + 1):
+                    ref_re[j] = ref_re[j - period]
+                    ref_im[j] = ref_im[j - period]
+                return ref_re, ref_im, max_iter
+            X2 = (X * X) >> sh
+            Y2 = (Y * Y) >> sh
+            Y = ((X * Y) >> (sh - 1)) + CY
+            X = X2 - Y2 + CX
+    elif fractal_type in (1, 3):
+        for k in range(max_iter):
+            rx, ry = X / one, Y / one
+            ref_re[k], ref_im[k] = rx, ry
+            if rx * rx + ry * ry > 65536.0:
+                ref_len = k
+                break
+            X2 = (X * X) >> sh
+            Y2 = (Y * Y) >> sh
+            Y = ((abs(X) * abs(Y)) >> (sh - 1)) - CY
+            X = X2 - Y2 + CX
+    else:
+        for k in range(max_iter):
+            rx, ry = X / one, Y / one
+            ref_re[k], ref_im[k] = rx, ry
+            if rx * rx + ry * ry > 65536.0:
+                ref_len = k
+                break
+            pr, pi = X, Y
+            for _ in range(gen_n - 1):
+                pr, pi = (pr * X - pi * Y) >> sh, (pr * Y + pi * X) >> sh
+            kzr = (KR * X - KI * Y) >> sh
+            kzi = (KR * Y + KI * X) >> sh
+            X, Y = pr + kzr + CX, pi + kzi + CY
+    ref_re[ref_len] = X / one
+    ref_im[ref_len] = Y / one
+    return ref_re, ref_im, ref_len
