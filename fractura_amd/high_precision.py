@@ -9,7 +9,9 @@ period detection on exact equality at doubling checkpoints for types 0/2,
 Z_ref_len stored after the loop) follows the reference; only the rounding of
 the big-number arithmetic differs, far below the stored double's precision.
 """
+import os
 import threading
+from concurrent.futures import ThreadPoolExecutor
 from decimal import Decimal
 
 import numpy as np
@@ -222,13 +224,21 @@ def find_optimal_reference_orbit(cx_dec, cy_dec, pw_dec, ph_dec, max_iter, scree
             cx_dec, cy_dec, max_iter, fractal_type, julia_cx, julia_cy, bits, gen_n, gen_kr, gen_ki)
         best = (cx_dec, cy_dec, 0.5, 0.5)
         if probe and ref_len < int(max_iter * 0.6) and pw_dec >= Decimal("1e-10000"):
-            results = []
-            for u, v in _PROBE_UV:
-                px = x_min + Decimal(str(u)) * pw_dec
-                py = y_max - Decimal(str(v)) * ph_dec
-                esc = compute_highprec_escape_scalar(px, py, max_iter, fractal_type, julia_cx,
-                                                     julia_cy, bits, gen_n, gen_kr, gen_ki)
-                results.append((esc, px, py, u, v))
+            # The 16 probe orbits run in threads, as the reference's do: the
+            # library's orbit routine runs without the interpreter lock.  The
+            # points are formed here, under this thread's Decimal context.
+            pts = [(x_min + Decimal(str(u)) * pw_dec, y_max - Decimal(str(v)) * ph_dec, u, v)
+                   for u, v in _PROBE_UV]
+
+            def escape_of(pt):
+                return compute_highprec_escape_scalar(pt[0], pt[1], max_iter, fractal_type,
+                                                      julia_cx, julia_cy, bits, gen_n, gen_kr,
+                                                      gen_ki)
+
+            workers = max(1, min(len(pts), os.cpu_count() or 1))
+            with ThreadPoolExecutor(max_workers=workers) as ex:
+                escs = list(ex.map(escape_of, pts))
+            results = [(esc, px, py, u, v) for esc, (px, py, u, v) in zip(escs, pts)]
             best_esc = max(int(ref_len * 1.2), ref_len + 32)
             cand = None
             for esc, px, py, u, v in results:
