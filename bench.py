@@ -520,9 +520,8 @@ def run_fp64(args, torch):
 
 def prepare_flythrough(args):
     """Host side of the config-5 fly-through, before the GPU is touched: the
-    64 frames' reference orbits, computed in a forked process pool (or in this
-    process with --orbit-processes 1, for runs under a profiler, whose
-    preloaded library the forked workers would inherit)."""
+    64 frames' reference orbits, computed by a thread pool (--orbit-processes
+    workers; 1 plans them in turn)."""
     from fractura_amd import batch
 
     cfg5 = load_configs()["config5"]
@@ -664,7 +663,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--orbit-processes", type=int, default=0,
-                    help="fly-through: host orbit processes (0 = one per core, up to 16)")
+                    help="fly-through: host orbit threads (0 = one per core, up to 16)")
     ap.add_argument("--in-flight", type=int, default=2,
                     help="config 3: frames in flight (stream slots) in the timed loop")
     ap.add_argument("--tile", type=int, nargs=2, default=None,
@@ -678,7 +677,7 @@ def main():
         sys.exit(2)
     import torch
 
-    # The fly-through's host orbits come from a forked pool: before the GPU is touched.
+    # The fly-through's host orbits are planned before the GPU is touched.
     prepared = prepare_flythrough(args) if args.workload == "flythrough" else None
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device"}))

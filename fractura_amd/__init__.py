@@ -33,8 +33,8 @@ __all__ = [
 
 def __getattr__(name):
     # HAS_CUDA (math_core/cuda_kernels.py) is answered on first use, so that
-    # importing the package does not start the GPU runtime (callers fork
-    # orbit workers before touching it).
+    # importing the package does not start the GPU runtime (a caller may
+    # still fork worker processes after the import).
     if name == "HAS_CUDA":
         from .device import has_gpu
 

@@ -11,8 +11,8 @@ parameters follow its zoom-in focal tracking (workers/video_workers.py:67-80).
 """
 import ctypes as C
 import json
-import multiprocessing as mp
 import os
+from concurrent.futures import ThreadPoolExecutor
 from decimal import Decimal
 
 import numpy as np
@@ -40,16 +40,16 @@ def frame_plan(spec):
 def plan_frames(specs, processes=None):
     """Plans of many frames, their host orbits computed in parallel.
 
-    The orbits are CPU-bound big-integer work, one process per frame; the pool
-    forks, so call this before the process touches the GPU."""
-    procs = processes or min(16, os.cpu_count() or 1, len(specs))
-    if procs <= 1 or len(specs) <= 1:
+    The orbits are CPU-bound big-integer work in the library's orbit routine,
+    which runs without the interpreter lock, so the frames are planned by a
+    pool of threads (`processes` of them; default one per core, up to 16),
+    deepest frames first since they take longest."""
+    workers = processes or min(16, os.cpu_count() or 1, len(specs))
+    if workers <= 1 or len(specs) <= 1:
         return [frame_plan(s) for s in specs]
-    ctx = mp.get_context("fork")
-    with ctx.Pool(procs) as pool:
-        # Deepest frames first: they take longest.
-        order = sorted(range(len(specs)), key=lambda i: -int(specs[i]["max_iter"]))
-        plans = pool.map(frame_plan, [specs[i] for i in order], chunksize=1)
+    order = sorted(range(len(specs)), key=lambda i: -int(specs[i]["max_iter"]))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        plans = list(ex.map(frame_plan, [specs[i] for i in order]))
     out = [None] * len(specs)
     for i, p in zip(order, plans):
         out[i] = p
