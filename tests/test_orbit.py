@@ -166,3 +166,13 @@ def test_entry_point_rejects_bad_arguments():
     assert lib.ap_reference_orbit(*args(0, 0, 2)) == _abi.AP_ERR_ARG       # limbs
     assert lib.ap_reference_orbit(*args(0, _abi.AP_MAX_ORBIT_LIMBS + 1, 2)) == _abi.AP_ERR_ARG
     assert lib.ap_reference_orbit(0, 10, 320, 1, None, None, 2, None, None, None) == _abi.AP_ERR_ARG
+
+
+@pytest.mark.parametrize("max_iter", [0, 1, 2, 8, 9, 17])
+def test_short_orbits_every_type(max_iter):
+    """max_iter at and around the first period checkpoints (k = 8, 16)."""
+    for ftype in range(5):
+        kw = dict(jx=0.3, jy=-0.2) if ftype in (2, 3) else {}
+        got = compute_single_highprec_orbit(Decimal("-0.1"), Decimal("0.65"), max_iter, ftype,
+                                            kw.get("jx", 0.0), kw.get("jy", 0.0), 256)
+        _same(got, _restated(Decimal("-0.1"), Decimal("0.65"), max_iter, ftype, **kw))
