@@ -13,8 +13,9 @@
 // 749-761) and the store of Z_ref_len after the loop follow the reference.
 //
 // Numbers are sign + magnitude over L 64-bit limbs, little endian; products
-// are formed in 2L + 1 limbs (schoolbook by columns with a 192-bit
-// accumulator; a square forms each cross product once).
+// are formed in 2L + 1 limbs (Karatsuba above 40 limbs, below it schoolbook
+// by columns with a 192-bit accumulator; a square forms each cross product
+// once).
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
@@ -109,42 +110,114 @@ struct Acc {
     }
 };
 
-// r (size >= |a| + |b| limbs) = |a| * |b|, by columns (product scanning):
-// the multiplies of a column are independent of each other, so they issue
-// back to back instead of waiting on a row's carry chain.  A square sums each
-// cross product of a column once and doubles the sum.
+// r[0, na + nb) = a[0, na) * b[0, nb) by columns (product scanning): the
+// multiplies of a column are independent of each other, so they issue back
+// to back instead of waiting on a row's carry chain.  sq: a == b (each cross
+// product of a column is formed once and the sum doubled).
+void comba(uint64_t* r, const uint64_t* a, int na, const uint64_t* b, int nb, bool sq) {
+    Acc acc;
+    const int ncol = na + nb - 1;
+    if (sq) {
+        for (int k = 0; k < ncol; ++k) {
+            const int i0 = k - na + 1 > 0 ? k - na + 1 : 0;
+            Acc cross;
+            for (int i = i0; 2 * i < k; ++i) cross.add((u128)a[i] * a[k - i]);
+            // acc += 2 * cross (+ the diagonal square for even k)
+            acc.add(((u128)cross.w1 << 65) | ((u128)cross.w0 << 1));
+            acc.w2 += (cross.w2 << 1) | (cross.w1 >> 63);
+            if ((k & 1) == 0) acc.add((u128)a[k >> 1] * a[k >> 1]);
+            r[k] = acc.take();
+        }
+    } else {
+        for (int k = 0; k < ncol; ++k) {
+            const int i0 = k - nb + 1 > 0 ? k - nb + 1 : 0;
+            const int i1 = k < na - 1 ? k : na - 1;
+            for (int i = i0; i <= i1; ++i) acc.add((u128)a[i] * b[k - i]);
+            r[k] = acc.take();
+        }
+    }
+    r[ncol] = acc.take();
+}
+
+// r[0, n) += a[0, m) (m <= n), carries propagated to r's end; returns the
+// carry out of r.
+uint64_t add_into(uint64_t* r, int n, const uint64_t* a, int m) {
+    unsigned long long c = 0;
+    int i = 0;
+    for (; i < m; ++i) r[i] = __builtin_addcll(r[i], a[i], c, &c);
+    for (; i < n && c; ++i) r[i] = __builtin_addcll(r[i], 0, c, &c);
+    return c;
+}
+
+// r[0, n) -= a[0, m) (m <= n, the result non-negative).
+void sub_from(uint64_t* r, int n, const uint64_t* a, int m) {
+    unsigned long long c = 0;
+    int i = 0;
+    for (; i < m; ++i) r[i] = __builtin_subcll(r[i], a[i], c, &c);
+    for (; i < n && c; ++i) r[i] = __builtin_subcll(r[i], 0, c, &c);
+}
+
+// Below this many limbs the column product is faster than a Karatsuba split.
+constexpr int kKaratsuba = 40;
+
+// r[0, 2n) = a[0, n) * b[0, n) (sq: a == b), Karatsuba: with a = a1 B^h + a0,
+// a b = a1 b1 B^2h + ((a0 + a1)(b0 + b1) - a0 b0 - a1 b1) B^h + a0 b0, every
+// part exact.  t: scratch of at least 6 (n + 2) limbs per level.
+void kmul(uint64_t* r, const uint64_t* a, const uint64_t* b, int n, bool sq, uint64_t* t) {
+    if (n < kKaratsuba) {
+        comba(r, a, n, b, n, sq);
+        return;
+    }
+    const int h = n / 2, m = n - h;  // low h limbs, high m >= h limbs
+    kmul(r, a, b, h, sq, t);                  // a0 b0 -> r[0, 2h)
+    kmul(r + 2 * h, a + h, b + h, m, sq, t);  // a1 b1 -> r[2h, 2n)
+    const int s = m + 1;
+    uint64_t* sa = t;
+    uint64_t* sb = t + s;
+    uint64_t* mid = t + 2 * s;  // 2s limbs
+    memcpy(sa, a + h, m * sizeof(uint64_t));
+    sa[m] = 0;
+    add_into(sa, s, a, h);
+    if (!sq) {
+        memcpy(sb, b + h, m * sizeof(uint64_t));
+        sb[m] = 0;
+        add_into(sb, s, b, h);
+    }
+    kmul(mid, sa, sq ? sa : sb, s, sq, t + 4 * s);
+    sub_from(mid, 2 * s, r, 2 * h);
+    sub_from(mid, 2 * s, r + 2 * h, 2 * m);
+    // mid < 2^(64 (n + 2)); add it at limb h (the total fits in 2n limbs).
+    int len = 2 * s;
+    while (len > 0 && mid[len - 1] == 0) --len;
+    add_into(r + h, 2 * n - h, mid, len);
+}
+
+// r (size >= |a| + |b| limbs) = |a| * |b|.  Operands of similar length above
+// the Karatsuba threshold (the orbit's squares and products) split; others go
+// by columns.
 void mul_mag(std::vector<uint64_t>& r, const std::vector<uint64_t>& a, const std::vector<uint64_t>& b) {
     const int na = top_limb(a) + 1, nb = top_limb(b) + 1;
     if (na == 0 || nb == 0) {
         std::fill(r.begin(), r.end(), 0);
         return;
     }
-    const uint64_t* pa = a.data();
-    const uint64_t* pb = b.data();
-    uint64_t* pr = r.data();
-    Acc acc;
-    const int ncol = na + nb - 1;
-    if (&a == &b) {
-        for (int k = 0; k < ncol; ++k) {
-            const int i0 = k - na + 1 > 0 ? k - na + 1 : 0;
-            Acc cross;
-            for (int i = i0; 2 * i < k; ++i) cross.add((u128)pa[i] * pa[k - i]);
-            // acc += 2 * cross (+ the diagonal square for even k)
-            acc.add(((u128)cross.w1 << 65) | ((u128)cross.w0 << 1));
-            acc.w2 += (cross.w2 << 1) | (cross.w1 >> 63);
-            if ((k & 1) == 0) acc.add((u128)pa[k >> 1] * pa[k >> 1]);
-            pr[k] = acc.take();
-        }
+    const bool sq = &a == &b;
+    const int n = na > nb ? na : nb;
+    int done;
+    const bool balanced = 2 * na > n && 2 * nb > n;  // a short factor (type 4's k) goes by columns
+    if (n >= kKaratsuba && balanced && (size_t)n <= a.size() && (size_t)n <= b.size() &&
+        2 * (size_t)n <= r.size()) {
+        // Both operands as n limbs (the shorter one's high limbs are zero).
+        static thread_local std::vector<uint64_t> scratch;
+        const size_t need = 16 * ((size_t)n + 8);
+        if (scratch.size() < need) scratch.resize(need);
+        kmul(r.data(), a.data(), b.data(), n, sq, scratch.data());
+        done = 2 * n;
     } else {
-        for (int k = 0; k < ncol; ++k) {
-            const int i0 = k - nb + 1 > 0 ? k - nb + 1 : 0;
-            const int i1 = k < na - 1 ? k : na - 1;
-            for (int i = i0; i <= i1; ++i) acc.add((u128)pa[i] * pb[k - i]);
-            pr[k] = acc.take();
-        }
+        comba(r.data(), a.data(), na, b.data(), nb, sq);
+        done = na + nb;
     }
-    pr[ncol] = acc.take();
-    std::fill(r.begin() + ncol + 1, r.end(), 0);
+    std::fill(r.begin() + done, r.end(), 0);
 }
 
 // Signed product of a and b into a wide number.

@@ -4,7 +4,8 @@ behind compute_single_highprec_orbit (math_core/high_precision.py:706-875).
 Pins:
   * the native routine equals the Python fixed-point restatement
     (oracle.highprec_orbit) bit for bit (stored doubles, ref_len, the periodic
-    fill) on seeded points of every type, at 320 to 3513 fraction bits,
+    fill) on seeded points of every type, at 320 to 3513 fraction bits, and at
+    2500 to 33300 bits where the products split (Karatsuba),
     including values that round to subnormal doubles and orbits that escape
     on the first steps;
   * the restatement, and with it the routine, against mpmath at the same
@@ -176,3 +177,14 @@ def test_short_orbits_every_type(max_iter):
         got = compute_single_highprec_orbit(Decimal("-0.1"), Decimal("0.65"), max_iter, ftype,
                                             kw.get("jx", 0.0), kw.get("jy", 0.0), 256)
         _same(got, _restated(Decimal("-0.1"), Decimal("0.65"), max_iter, ftype, **kw))
+
+
+@pytest.mark.parametrize("bits", [2500, 10000, 33300])
+def test_wide_precisions_split_products(bits):
+    """Above 40 limbs the products split (Karatsuba, up to four levels at
+    33300 bits): still the restatement's orbit, bit for bit."""
+    cx = Decimal("-0.7436438870371587047521915061147745")
+    cy = Decimal("0.1318259042053119704931320563851390")
+    for ftype in (0, 1, 4):
+        got = compute_single_highprec_orbit(cx, cy, 60, ftype, prec_bits=bits)
+        _same(got, _restated(cx, cy, 60, ftype, bits=bits))
