@@ -16,10 +16,11 @@ Workloads (inputs seeded, tests/golden/configs.json):
         as one HIP graph.
 A step is one frame of device work with the inputs (the reference orbit)
 resident in HBM: the BLA table build from the orbit (deep), the pixel pass
-and the palette colouring, captured as one HIP graph and replayed.  The host
-orbit is computed once before timing.  Eager steps with each part bracketed
-by HIP events on the library's stream give the breakdown; the roofline
-figures use the pass kernel's own event time.
+and the palette colouring, captured as one HIP graph and replayed (deep: two
+frames in flight, one per stream slot).  The host orbit is computed once
+before timing.  Eager steps with each part bracketed by HIP events on the
+library's stream give the breakdown; the roofline figures use the pass
+kernel's own event time.
 
 Prints one JSON line.  value = Giter/s, an iteration being one effective
 escape-time iteration of one sample (BLA-skipped iterations count,
@@ -107,10 +108,11 @@ def time_graph(torch, graph, warmup, steps):
 
 def time_in_flight(torch, graphs, warmup, steps):
     """Wall time per frame with len(graphs) frames in flight: step i replays
-    graph i mod n on stream i mod n (each graph has its own table and output
-    buffers, as the reference's stream slots do), so one frame's latency-bound
-    table build and colouring overlap another's pixel pass.  Streams keep the
-    default priority; barrier + synchronize on both sides of the timed loop."""
+    the whole-frame graph i mod n on stream i mod n (each graph has its own
+    table and output buffers, as the reference's stream slots do).  The slots'
+    latency-bound table builds run side by side, and so do their colourings.
+    Streams keep the default priority; barrier + synchronize on both sides of
+    the timed loop."""
     n = len(graphs)
     cur = torch.cuda.current_stream()
     streams = [torch.cuda.Stream() for _ in range(n)]
@@ -656,7 +658,7 @@ def run_copy(args, torch):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["deep", "fp64", "flythrough", "copy"], default="deep")
     ap.add_argument("--size", type=int, nargs=2, default=None)
