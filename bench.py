@@ -7,7 +7,9 @@ Workloads (inputs seeded, tests/golden/configs.json):
         centre on the main-cardioid boundary (1100 digits), chained BLA
         (16 levels, tol 1e-6), Pauldelbrot glitch test.  Timed: the FP32-
         mantissa FloatExp path; the FP64-mantissa FloatExp path (the
-        reference's FloatExp design) is timed on the same orbit and table.
+        reference's FloatExp design) is timed on the same orbit and table,
+        and the plain FP64-delta path (PERT64) against the FP32 path at the
+        same centre at widths 1e-100 and 1e-250 (vs_fp64_delta).
   fp64  Plain FP64 Mandelbrot at (-0.65, 0), width 4, max_iter 1000,
         3840x2160 (config 1's view at 4K).
   flythrough  Config 5: 64 frames of z^3 + k z + c zooming from width 4 to
@@ -47,6 +49,7 @@ HBM_PEAK_GBS = 8000.0
 FLOPS_FP64_ITER = 8        # z^2 + c: 4 mul + 4 add
 FLOPS_FEXP_PLAIN = 21      # 2 Z dz (8) + dz^2 2^s (9) + dc 2^-e (4)
 FLOPS_BLA_STEP = 14        # A dz + B dc: 8 mul + 6 add
+FLOPS_PERT_PLAIN = 19      # FP64 delta: 2 Z dz + dz^2 + dc
 
 
 def log(msg):
@@ -248,13 +251,17 @@ def run_deep(args, torch):
             stats=torch.zeros(4, dtype=torch.int64, device=dev.torch_device))
             for _ in range(IN_FLIGHT)]
 
-    def step(name, plan, slot=0, timers=None, with_stats=False):
+    tables = {"fexp32": orbits, "fexp64": orbits}
+
+    def step(name, plan, slot=0, timers=None, with_stats=False, key=None):
         """BLA build, pixel pass, colouring of one frame on slot `slot`'s
         buffers: each part bracketed by its own HIP events (on the stream the
-        library launches on) when timed."""
+        library launches on) when timed.  `key` names the buffer set (default:
+        the engine's)."""
+        key = key or name
         v = plan.view.copy()
-        o = outs[name][slot]
-        orbit = orbits[slot]
+        o = outs[key][slot]
+        orbit = tables[key][slot]
         ao = engine._outs(o["counts"], o["smooth"], o["iter_map"], o["glitch"],
                           stats=o["stats"] if with_stats else None)
         t = timers or {}
@@ -268,46 +275,51 @@ def run_deep(args, torch):
             dev.call("ap_colorize", C.byref(v), engine.ptr(o["iter_map"]), None, 0,
                      engine.ptr(lut), engine.ptr(o["rgb"]))
 
-    results = {}
-    for name, plan in (("fexp32", plan32), ("fexp64", plan64)):
-        outs[name][0]["stats"].zero_()
-        step(name, plan, with_stats=True)
+    def measure(name, plan, key=None):
+        """Iteration statistics, eager breakdown, one-at-a-time and in-flight
+        wall time of one view on one engine."""
+        key = key or name
+        outs[key][0]["stats"].zero_()
+        step(name, plan, with_stats=True, key=key)
         torch.cuda.synchronize()
-        st = outs[name][0]["stats"].cpu().numpy().view(np.uint64)
+        st = outs[key][0]["stats"].cpu().numpy().view(np.uint64)
         iters, plain, bla = int(st[0]), int(st[1]), int(st[2])
         for _ in range(args.warmup):
-            step(name, plan)
+            step(name, plan, key=key)
         torch.cuda.synchronize()
         # Eager steps, each part bracketed by its own events: the breakdown
         # and the pass kernel's time for the roofline.
         timers = {k: Timer(torch) for k in ("build", "pass", "colour")}
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            step(name, plan, timers=timers)
+            step(name, plan, timers=timers, key=key)
         torch.cuda.synchronize()
         eager_wall = (time.perf_counter() - t0) / args.steps
         # The frame as one HIP graph per slot (build, pass, colouring): one
         # frame at a time on one stream, then IN_FLIGHT frames in flight (the
         # timed steps of `value`).
-        graphs = [engine.capture_graph(dev, lambda i=i: step(name, plan, slot=i))
+        graphs = [engine.capture_graph(dev, lambda i=i: step(name, plan, slot=i, key=key))
                   for i in range(IN_FLIGHT)]
         wall1, graph_ms = time_graph(torch, graphs[0], args.warmup, args.steps)
         wall = time_in_flight(torch, graphs, args.warmup, args.steps)
         # The slots rendered the same frame: their outputs must agree.
-        slots_equal = all(torch.equal(outs[name][0][k], outs[name][i][k])
+        slots_equal = all(torch.equal(outs[key][0][k], outs[key][i][k])
                           for i in range(1, IN_FLIGHT) for k in ("counts", "smooth", "rgb"))
         if not slots_equal:
-            log(f"{name}: the {IN_FLIGHT} slots' outputs differ")
+            log(f"{key}: the {IN_FLIGHT} slots' outputs differ")
         kern_ms = timers["pass"].mean_ms()
-        flops = plain * FLOPS_FEXP_PLAIN + bla * FLOPS_BLA_STEP
-        results[name] = dict(iters=iters, plain=plain, bla=bla, wall_ms=wall * 1e3,
-                             single_wall_ms=wall1 * 1e3, graph_ms=graph_ms,
-                             eager_wall_ms=eager_wall * 1e3, kernel_ms=kern_ms,
-                             build_ms=timers["build"].mean_ms(),
-                             colour_ms=timers["colour"].mean_ms(), giters=iters / wall / 1e9,
-                             single_giters=iters / wall1 / 1e9, slots_equal=slots_equal,
-                             tflops=flops / (kern_ms * 1e-3) / 1e12 if kern_ms > 0 else 0.0)
-        log(f"{name}: {results[name]}")
+        flops = plain * (FLOPS_PERT_PLAIN if name == "pert64" else FLOPS_FEXP_PLAIN) + bla * FLOPS_BLA_STEP
+        r = dict(iters=iters, plain=plain, bla=bla, wall_ms=wall * 1e3,
+                 single_wall_ms=wall1 * 1e3, graph_ms=graph_ms,
+                 eager_wall_ms=eager_wall * 1e3, kernel_ms=kern_ms,
+                 build_ms=timers["build"].mean_ms(),
+                 colour_ms=timers["colour"].mean_ms(), giters=iters / wall / 1e9,
+                 single_giters=iters / wall1 / 1e9, slots_equal=slots_equal,
+                 tflops=flops / (kern_ms * 1e-3) / 1e12 if kern_ms > 0 else 0.0)
+        log(f"{key}: {r}")
+        return r
+
+    results = {name: measure(name, plan) for name, plan in (("fexp32", plan32), ("fexp64", plan64))}
 
     c32 = outs["fexp32"][0]["counts"].cpu().numpy()
     c64 = outs["fexp64"][0]["counts"].cpu().numpy()
@@ -321,6 +333,9 @@ def run_deep(args, torch):
 
     # Oracle subsample: a 48x48 block of the same view on the CPU restatement.
     par = oracle_block(plan64, orbit_host=plan64.orbit, bla_tol=plan64.bla_tol, c64=c64, c32=c32)
+
+    delta = {w: fp64_delta_leg(measure, outs, tables, dev, torch, cfg, W, H, IN_FLIGHT, w)
+             for w in DELTA_WIDTHS}
 
     cpu = None
     if not args.no_cpu:
@@ -360,8 +375,67 @@ def run_deep(args, torch):
                         "eager_wall": r["eager_wall_ms"]}
                     for e, r in (("fexp32", r32), ("fexp64", r64))},
         "colour_pass": colour_roofline(n, r32["colour_ms"]),
+        "vs_fp64_delta": delta,
         "cpu_baseline": cpu,
     }
+
+
+# Two depths for the FP64-delta comparison: at 1e-100 the squares of FP64
+# deltas (pixel spacing ~3e-104) are normal numbers; at 1e-250 they underflow
+# to zero, and the delta path's table lookup then caps a leap at level 8
+# (math_core/cpu_kernels.py:438-439), so a pixel takes ~780 leaps instead of 14.
+DELTA_WIDTHS = ("1e-100", "1e-250")
+
+
+def fp64_delta_leg(measure, outs, tables, dev, torch, cfg, W, H, slots, width):
+    """The FP32 FloatExp path against the plain FP64-delta path (PERT64: FP64
+    deltas, the same chained BLA at tol 1e-6, no exponent management), timed
+    the same way, on a view both can render: the deep view's centre at
+    `width`.  Count agreement between the two frames, and of each against the
+    CPU restatement of its own engine on a centred 48x48 block."""
+    from fractura_amd import _abi, engine
+    from oracle import oracle
+
+    res = {}
+    for name in ("pert64", "fexp32"):
+        plan = engine.plan_view(W, H, cfg["max_iter"], Decimal(cfg["center_x"]),
+                                Decimal(cfg["center_y"]), Decimal(width), bbsa_order=32,
+                                bbsa_tol=1e-6, glitch_mode="Pauldelbrot", fexp_mantissa=32,
+                                engine_override=name)
+        key = f"{name}@{width}"
+        tables[key] = [engine._Orbit(dev, plan, 0, tag=f"delta{name}{i}", forms=_abi.AP_BLA_FORM_ALL)
+                       for i in range(slots)]
+        outs[key] = [{k: torch.empty_like(t) for k, t in outs["fexp32"][0].items()}
+                     for _ in range(slots)]
+        r = measure(name, plan, key=key)
+        counts = outs[key][0]["counts"].cpu().numpy()
+        # The same centred block on the CPU restatement of this engine.
+        v = plan.view.copy()
+        size = 48
+        x0, y0 = (W - size) // 2, (H - size) // 2
+        sub = v.copy()
+        sub.width, sub.height = size, size
+        sub.ref_px, sub.ref_py = v.ref_px - x0 * v.factor, v.ref_py - y0 * v.factor
+        re, im, ref_len = plan.orbit
+        bla = oracle.bla_build(re, im, ref_len, v.fractal_type, plan.bla_tol, 16)
+        m = oracle.render(name, sub, orbit=(re, im), bla=bla, levels=16)
+        r["oracle_block_exact"] = float(np.mean(counts[y0:y0 + size, x0:x0 + size] == m.counts))
+        r["counts"], r["smooth"] = counts, outs[key][0]["smooth"].cpu().numpy()
+        res[name] = r
+    p, f = res["pert64"], res["fexp32"]
+    esc = (p["smooth"] >= 0) & (f["smooth"] >= 0)
+    out = {"view": f"deep view centre, width {width}, {W}x{H}, max_iter {cfg['max_iter']}, "
+                   f"chained BLA d=0..15 tol 1e-6, {slots} frames in flight",
+           "count_agreement": float(np.mean(p["counts"] == f["counts"])),
+           "max_smooth_diff": float(np.max(np.abs(p["smooth"][esc] - f["smooth"][esc]), initial=0.0)),
+           "ratio": f["giters"] / p["giters"] if p["giters"] else None}
+    for name, r in res.items():
+        out[name] = {"value": r["giters"], "ms_per_step": r["wall_ms"], "pass_ms": r["kernel_ms"],
+                     "one_at_a_time_ms": r["single_wall_ms"], "iterations": r["iters"],
+                     "executed_steps": {"plain": r["plain"], "bla": r["bla"]},
+                     "oracle_block_exact": r["oracle_block_exact"]}
+    log(f"vs FP64 delta at {width}: {out}")
+    return out
 
 
 def colour_roofline(pixels, ms):
