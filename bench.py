@@ -334,8 +334,9 @@ def run_deep(args, torch):
     # Oracle subsample: a 48x48 block of the same view on the CPU restatement.
     par = oracle_block(plan64, orbit_host=plan64.orbit, bla_tol=plan64.bla_tol, c64=c64, c32=c32)
 
-    delta = {w: fp64_delta_leg(measure, outs, tables, dev, torch, cfg, W, H, IN_FLIGHT, w)
-             for w in DELTA_WIDTHS}
+    delta = None if args.no_delta else {
+        w: fp64_delta_leg(measure, outs, tables, dev, torch, cfg, W, H, IN_FLIGHT, w)
+        for w in DELTA_WIDTHS}
 
     cpu = None
     if not args.no_cpu:
@@ -737,6 +738,8 @@ def main():
     ap.add_argument("--workload", choices=["deep", "fp64", "flythrough", "copy"], default="deep")
     ap.add_argument("--size", type=int, nargs=2, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-delta", action="store_true",
+                    help="deep: skip the FP64-delta comparison (profiler runs of the deep frame alone)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--orbit-processes", type=int, default=0,
                     help="fly-through: host orbit threads (0 = one per core, up to 16)")

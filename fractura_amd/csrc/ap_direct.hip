@@ -18,8 +18,10 @@ __device__ inline void inv_ln_pair(const ap_view& v, double& ref, double& exact)
     }
 }
 
-// Escape test every iteration; the update is the reference's, written so the
-// compiler sees exactly its multiplies and adds.
+constexpr int kDirectUnroll = 4;  // FP64 iterations per escape test
+
+// Every iteration's |z|^2 is tested, as the reference tests it; the update is
+// the reference's, operation for operation.
 template <int T>
 __device__ inline Sample fp64_sample(const ap_view& v, double x, double y) {
     if (T == AP_MANDELBROT && in_main_components(x, y)) return interior(-1, 0);
@@ -34,34 +36,75 @@ __device__ inline Sample fp64_sample(const ap_view& v, double x, double y) {
     const int max_iter = v.max_iter;
     const double kr = v.gen_kr, ki = v.gen_ki;
     const int n = v.gen_n;
-    // One exit for both ends (escape at k < max_iter, or k = max_iter): the
-    // escape is re-tested after the loop from the same z, so the iterations
-    // carry nothing for the exit paths.  At k = max_iter |z|^2 is formed and
-    // not tested, as the reference returns there without testing it.
-    int k = 0;
-    for (;; ++k) {
-        double zx2 = zx * zx;
-        double zy2 = zy * zy;
-        double mag2 = zx2 + zy2;
-        if (k >= max_iter || mag2 > kBailout2) break;
+    // (ox, oy) = f(x, y), the reference's update written so the compiler sees
+    // exactly its multiplies and adds (x2 = x*x, y2 = y*y already formed).
+    auto step = [&](double x, double y, double x2, double y2, double& ox, double& oy) {
         if (T == AP_MANDELBROT || T == AP_JULIA_MANDELBROT) {
-            zy = 2.0 * zx * zy + cy;
-            zx = zx2 - zy2 + cx;
+            oy = 2.0 * x * y + cy;
+            ox = x2 - y2 + cx;
         } else if (T == AP_BURNING_SHIP || T == AP_JULIA_BURNING_SHIP) {
-            zy = 2.0 * fabs(zx) * fabs(zy) - cy;
-            zx = zx2 - zy2 + cx;
+            oy = 2.0 * fabs(x) * fabs(y) - cy;
+            ox = x2 - y2 + cx;
         } else {
-            double pr = zx, pi = zy;
+            double pr = x, pi = y;
             for (int e = 0; e < n - 1; ++e) {
-                double t = pr * zx - pi * zy;
-                pi = pr * zy + pi * zx;
+                double t = pr * x - pi * y;
+                pi = pr * y + pi * x;
                 pr = t;
             }
-            double kzr = kr * zx - ki * zy;
-            double kzi = kr * zy + ki * zx;
-            zx = pr + kzr + cx;
-            zy = pi + kzi + cy;
+            double kzr = kr * x - ki * y;
+            double kzi = kr * y + ki * x;
+            ox = pr + kzr + cx;
+            oy = pi + kzi + cy;
         }
+    };
+    // kDirectUnroll iterations per trip with one escape test, on the max of
+    // their |z|^2, and one loop exit shared by the escape and the bound: the
+    // exec-mask bookkeeping (scalar instructions that otherwise outnumber the
+    // FP64 arithmetic) is paid once per trip.  v_max_f64 drops a NaN operand,
+    // so the test equals (m[0] > B || m[1] > B || ...), the reference's
+    // tests; steps past an escape may overflow to inf or NaN and are unused.
+    // The trip works on a copy of z: on the bound exit z stays z_k for the
+    // last steps below.  After the loop, k + kDirectUnroll <= max_iter says
+    // the exit was an escape, at k + j for the first m[j] > B.
+    int k = 0;
+    double m[kDirectUnroll];
+    for (;;) {
+        double px = zx, py = zy, x2 = 0.0, y2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < kDirectUnroll; ++j) {
+            x2 = px * px;
+            y2 = py * py;
+            m[j] = x2 + y2;
+            if (j + 1 < kDirectUnroll) step(px, py, x2, y2, px, py);
+        }
+        double mm = m[0];
+#pragma unroll
+        for (int j = 1; j < kDirectUnroll; ++j) mm = fmax(mm, m[j]);
+        if (k + kDirectUnroll > max_iter || mm > kBailout2) break;
+        step(px, py, x2, y2, zx, zy);
+        k += kDirectUnroll;
+    }
+    if (k + kDirectUnroll <= max_iter) {
+        int j = kDirectUnroll - 1;
+        double mj = m[kDirectUnroll - 1];
+#pragma unroll
+        for (int i = kDirectUnroll - 2; i >= 0; --i) {
+            if (m[i] > kBailout2) {
+                j = i;
+                mj = m[i];
+            }
+        }
+        return escaped(k + j, mj, inv_ref, inv_exact, v.ln_bailout_log);
+    }
+    // The last max_iter - k < kDirectUnroll steps.  One exit for both ends:
+    // at k = max_iter |z|^2 is formed and not tested, as the reference
+    // returns there without testing it.
+    for (;; ++k) {
+        const double zx2 = zx * zx, zy2 = zy * zy;
+        const double mag2 = zx2 + zy2;
+        if (k >= max_iter || mag2 > kBailout2) break;
+        step(zx, zy, zx2, zy2, zx, zy);
     }
     if (k < max_iter) return escaped(k, zx * zx + zy * zy, inv_ref, inv_exact, v.ln_bailout_log);
     return interior(max_iter, max_iter);

@@ -60,6 +60,24 @@ def test_fp64_types(ftype):
     _check_maps(fr, m)
 
 
+@pytest.mark.parametrize("max_iter", [1, 2, 3, 5, 6, 7, 13, 101, 1002, 1003])
+def test_fp64_max_iter_remainders(max_iter):
+    """The FP64 loop tests escape once per four iterations and finishes the
+    last max_iter mod 4 steps one at a time: every remainder, every type,
+    bit-exact."""
+    from fractura_amd.config import FRACTAL_DEFAULTS
+
+    for ftype in range(5):
+        cx, cy, w = ("-0.65", "0.0", "4.0") if ftype == 0 else FRACTAL_DEFAULTS[ftype]
+        plan = engine.plan_view(96, 54, max_iter, cx, cy, w, fractal_type=ftype, julia_cx=-0.67,
+                                julia_cy=0.37, gen_n=3 + ftype % 2, gen_kr=0.25, gen_ki=1.0)
+        assert plan.engine == "fp64"
+        fr = engine.render_view(plan, "magma", rebase=False)
+        _sync()
+        m = oracle_frame(plan, lut(name="magma"))
+        _check_maps(fr, m, rgb_exact_frac=0.99)
+
+
 @pytest.mark.parametrize("scheme", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 def test_fp64_palettes(scheme):
     plan = config1_plan(256, 144, 300, scheme=scheme, palette_offset=0.3, color_density=1.7,
