@@ -157,7 +157,9 @@ def valu_issue(kernel, kernel_ms):
     """The kernel's VALU instruction count from the committed profiler summary
     (SQ_INSTS_VALU / SQ_WAVES, profiles/issue.json) as issue time: the time
     the kernel would take if every SIMD issued one VALU instruction every 4
-    cycles, and that time over the kernel's measured duration."""
+    cycles, and that time over the kernel's measured duration; with the VALU
+    lane efficiency (active lanes per VALU instruction, SQ_THREAD_CYCLES_VALU /
+    SQ_ACTIVE_INST_VALU, over 64)."""
     path = os.path.join(ROOT, "profiles", "issue.json")
     try:
         with open(path) as f:
@@ -168,7 +170,10 @@ def valu_issue(kernel, kernel_ms):
         return None
     ms = c["valu_per_wave"] * c["waves"] * CYCLES_PER_VALU / SIMDS / CLOCK_HZ * 1e3
     return {"valu_per_wave": c["valu_per_wave"], "waves": c["waves"], "issue_ms": ms,
-            "frac": ms / kernel_ms if kernel_ms else None, "source": c["source"]}
+            "frac": ms / kernel_ms if kernel_ms else None,
+            "active_lanes": c.get("active_lanes"), "lane_efficiency":
+                c["active_lanes"] / 64.0 if c.get("active_lanes") else None,
+            "source": c["source"]}
 
 
 def roofline(bound, kernel, achieved, peak, unit, kernel_ms, **extra):
