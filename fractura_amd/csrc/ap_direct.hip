@@ -19,6 +19,7 @@ __device__ inline void inv_ln_pair(const ap_view& v, double& ref, double& exact)
 }
 
 constexpr int kDirectUnroll = 4;  // FP64 iterations per escape test
+constexpr int kDDUnroll = 2;      // double-double iterations per escape test
 
 // Every iteration's |z|^2 is tested, as the reference tests it; the update is
 // the reference's, operation for operation.
@@ -137,38 +138,74 @@ __device__ inline Sample dd_sample(const ap_view& v, dd cx_in, dd cy_in) {
     inv_ln_pair(v, inv_ref, inv_exact);
     const int max_iter = v.max_iter;
     const int n = v.gen_n;
-    // One exit for both ends, as in the FP64 loop (the escape test is on the
-    // high words only, math_core/cpu_kernels.py:292-295).
+    // (ox, oy) = f(x, y) in double-double, the reference's operations
+    // (zx2 = x^2, zy2 = y^2 already formed).
+    auto step = [&](dd x, dd y, dd x2, dd y2, dd& ox, dd& oy) {
+        if (T == AP_MANDELBROT || T == AP_JULIA_MANDELBROT) {
+            dd sub = dd_sub(x2, y2);
+            dd xy = dd_mul(x, y);
+            ox = dd_add(sub, cx);
+            oy = dd_add(dd{xy.hi * 2.0, xy.lo * 2.0}, cy);
+        } else if (T == AP_BURNING_SHIP || T == AP_JULIA_BURNING_SHIP) {
+            dd sub = dd_sub(x2, y2);
+            dd xy = dd_mul(dd_abs(x), dd_abs(y));
+            ox = dd_add(sub, cx);
+            oy = dd_sub(dd{xy.hi * 2.0, xy.lo * 2.0}, cy);
+        } else {
+            dd pr = x, pi = y;
+            for (int e = 0; e < n - 1; ++e) {
+                dd re = dd_sub(dd_mul(pr, x), dd_mul(pi, y));
+                dd im = dd_add(dd_mul(pr, y), dd_mul(pi, x));
+                pr = re;
+                pi = im;
+            }
+            dd kzr = dd_sub(dd_mul_f64(x, v.gen_kr), dd_mul_f64(y, v.gen_ki));
+            dd kzi = dd_add(dd_mul_f64(y, v.gen_kr), dd_mul_f64(x, v.gen_ki));
+            ox = dd_add(dd_add(pr, kzr), cx);
+            oy = dd_add(dd_add(pi, kzi), cy);
+        }
+    };
+    // kDDUnroll steps per escape test and one shared exit, as in the FP64
+    // loop (the test is on the high words only,
+    // math_core/cpu_kernels.py:292-295).
     int k = 0;
+    double m[kDDUnroll];
+    for (;;) {
+        dd px = zx, py = zy, x2 = {0.0, 0.0}, y2 = {0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < kDDUnroll; ++j) {
+            x2 = dd_sqr(px);
+            y2 = dd_sqr(py);
+            m[j] = x2.hi + y2.hi;
+            if (j + 1 < kDDUnroll) step(px, py, x2, y2, px, py);
+        }
+        double mm = m[0];
+#pragma unroll
+        for (int j = 1; j < kDDUnroll; ++j) mm = fmax(mm, m[j]);
+        if (k + kDDUnroll > max_iter || mm > kBailout2) break;
+        step(px, py, x2, y2, zx, zy);
+        k += kDDUnroll;
+    }
+    if (k + kDDUnroll <= max_iter) {
+        int j = kDDUnroll - 1;
+        double mj = m[kDDUnroll - 1];
+#pragma unroll
+        for (int i = kDDUnroll - 2; i >= 0; --i) {
+            if (m[i] > kBailout2) {
+                j = i;
+                mj = m[i];
+            }
+        }
+        return escaped(k + j, mj, inv_ref, inv_exact, v.ln_bailout_log);
+    }
+    // The last max_iter - k < kDDUnroll steps, one exit for both ends.
     double mag2;
     for (;; ++k) {
         dd zx2 = dd_sqr(zx);
         dd zy2 = dd_sqr(zy);
         mag2 = zx2.hi + zy2.hi;
         if (k >= max_iter || mag2 > kBailout2) break;
-        if (T == AP_MANDELBROT || T == AP_JULIA_MANDELBROT) {
-            dd sub = dd_sub(zx2, zy2);
-            dd zxy = dd_mul(zx, zy);
-            zx = dd_add(sub, cx);
-            zy = dd_add(dd{zxy.hi * 2.0, zxy.lo * 2.0}, cy);
-        } else if (T == AP_BURNING_SHIP || T == AP_JULIA_BURNING_SHIP) {
-            dd sub = dd_sub(zx2, zy2);
-            dd zxy = dd_mul(dd_abs(zx), dd_abs(zy));
-            zx = dd_add(sub, cx);
-            zy = dd_sub(dd{zxy.hi * 2.0, zxy.lo * 2.0}, cy);
-        } else {
-            dd pr = zx, pi = zy;
-            for (int e = 0; e < n - 1; ++e) {
-                dd re = dd_sub(dd_mul(pr, zx), dd_mul(pi, zy));
-                dd im = dd_add(dd_mul(pr, zy), dd_mul(pi, zx));
-                pr = re;
-                pi = im;
-            }
-            dd kzr = dd_sub(dd_mul_f64(zx, v.gen_kr), dd_mul_f64(zy, v.gen_ki));
-            dd kzi = dd_add(dd_mul_f64(zy, v.gen_kr), dd_mul_f64(zx, v.gen_ki));
-            zx = dd_add(dd_add(pr, kzr), cx);
-            zy = dd_add(dd_add(pi, kzi), cy);
-        }
+        step(zx, zy, zx2, zy2, zx, zy);
     }
     if (k < max_iter) return escaped(k, mag2, inv_ref, inv_exact, v.ln_bailout_log);
     return interior(max_iter, max_iter);

@@ -122,3 +122,20 @@ def test_dd_types(ftype):
     _sync()
     m = oracle_frame(plan, lut())
     _check_maps(fr, m)
+
+
+@pytest.mark.parametrize("max_iter", [1, 2, 3, 7, 101])
+def test_dd_max_iter_remainders(max_iter):
+    """The DD loop tests escape once per two iterations and finishes an odd
+    last step alone: odd and even max_iter, every type, bit-exact."""
+    from fractura_amd.config import FRACTAL_DEFAULTS
+
+    for ftype in range(5):
+        cx, cy, w = ("-0.65", "0.0", "4.0") if ftype == 0 else FRACTAL_DEFAULTS[ftype]
+        plan = engine.plan_view(96, 54, max_iter, cx, cy, w, fractal_type=ftype, julia_cx=-0.67,
+                                julia_cy=0.37, gen_n=3 + ftype % 2, gen_kr=0.25, gen_ki=1.0,
+                                engine_override="dd")
+        fr = engine.render_view(plan, "magma", rebase=False)
+        _sync()
+        m = oracle_frame(plan, lut(name="magma"))
+        _check_maps(fr, m, rgb_exact_frac=0.99)
