@@ -62,9 +62,10 @@ __device__ inline Sample fp64_sample(const ap_view& v, double x, double y) {
     // kDirectUnroll iterations per trip with one escape test, on the max of
     // their |z|^2, and one loop exit shared by the escape and the bound: the
     // exec-mask bookkeeping (scalar instructions that otherwise outnumber the
-    // FP64 arithmetic) is paid once per trip.  v_max_f64 drops a NaN operand,
-    // so the test equals (m[0] > B || m[1] > B || ...), the reference's
-    // tests; steps past an escape may overflow to inf or NaN and are unused.
+    // FP64 arithmetic) is paid once per trip.  fmax returns the other operand
+    // when one is NaN, so the test equals (m[0] > B || m[1] > B || ...), the
+    // reference's tests; steps past an escape may overflow to inf or NaN and
+    // are unused.
     // The trip works on a copy of z: on the bound exit z stays z_k for the
     // last steps below.  After the loop, k + kDirectUnroll <= max_iter says
     // the exit was an escape, at k + j for the first m[j] > B.
