@@ -340,8 +340,9 @@ def run_deep(args, torch):
     par = oracle_block(plan64, orbit_host=plan64.orbit, bla_tol=plan64.bla_tol, c64=c64, c32=c32)
 
     delta = None if args.no_delta else {
-        w: fp64_delta_leg(measure, outs, tables, dev, torch, cfg, W, H, IN_FLIGHT, w)
-        for w in DELTA_WIDTHS}
+        f"{label}, width {w}": fp64_delta_leg(measure, outs, tables, dev, torch, W, H, IN_FLIGHT,
+                                              label, cx, cy, w, mi)
+        for label, cx, cy, w, mi in delta_views(cfg)}
 
     cpu = None
     if not args.no_cpu:
@@ -386,30 +387,37 @@ def run_deep(args, torch):
     }
 
 
-# Two depths for the FP64-delta comparison: at 1e-100 the squares of FP64
-# deltas (pixel spacing ~3e-104) are normal numbers; at 1e-250 they underflow
-# to zero, and the delta path's table lookup then caps a leap at level 8
-# (math_core/cpu_kernels.py:438-439), so a pixel takes ~780 leaps instead of 14.
-DELTA_WIDTHS = ("1e-100", "1e-250")
+def delta_views(cfg):
+    """Views of the FP64-delta comparison: the deep view's centre at two
+    depths, and c = i (a Misiurewicz point, where pixels escape over a spread
+    of counts).  At 1e-100 the squares of FP64 deltas (pixel spacing ~3e-104)
+    are normal numbers; at 1e-250 they underflow to zero, and the delta
+    path's table lookup then caps a leap at level 8
+    (math_core/cpu_kernels.py:438-439), so a pixel takes ~780 leaps instead
+    of 14."""
+    return (("deep centre", cfg["center_x"], cfg["center_y"], "1e-100", cfg["max_iter"]),
+            ("deep centre", cfg["center_x"], cfg["center_y"], "1e-250", cfg["max_iter"]),
+            ("c = i", "0", "1", "1e-100", 20000))
 
 
-def fp64_delta_leg(measure, outs, tables, dev, torch, cfg, W, H, slots, width):
+def fp64_delta_leg(measure, outs, tables, dev, torch, W, H, slots, label, cx, cy, width, max_iter):
     """The FP32 FloatExp path against the plain FP64-delta path (PERT64: FP64
     deltas, the same chained BLA at tol 1e-6, no exponent management), timed
-    the same way, on a view both can render: the deep view's centre at
-    `width`.  Count agreement between the two frames, and of each against the
-    CPU restatement of its own engine on a centred 48x48 block."""
+    the same way, on a view both can render (centre (cx, cy), `width`).
+    Count agreement and smooth difference between the two frames, and each
+    frame's counts against the CPU restatement of its own engine on a centred
+    48x48 block."""
     from fractura_amd import _abi, engine
     from oracle import oracle
 
     res = {}
     for name in ("pert64", "fexp32"):
-        plan = engine.plan_view(W, H, cfg["max_iter"], Decimal(cfg["center_x"]),
-                                Decimal(cfg["center_y"]), Decimal(width), bbsa_order=32,
+        plan = engine.plan_view(W, H, max_iter, Decimal(cx), Decimal(cy), Decimal(width), bbsa_order=32,
                                 bbsa_tol=1e-6, glitch_mode="Pauldelbrot", fexp_mantissa=32,
                                 engine_override=name)
-        key = f"{name}@{width}"
-        tables[key] = [engine._Orbit(dev, plan, 0, tag=f"delta{name}{i}", forms=_abi.AP_BLA_FORM_ALL)
+        key = f"{name}@{label}@{width}"
+        tables[key] = [engine._Orbit(dev, plan, 0, tag=f"delta{name}{label}{width}{i}",
+                                     forms=_abi.AP_BLA_FORM_ALL)
                        for i in range(slots)]
         outs[key] = [{k: torch.empty_like(t) for k, t in outs["fexp32"][0].items()}
                      for _ in range(slots)]
@@ -430,17 +438,20 @@ def fp64_delta_leg(measure, outs, tables, dev, torch, cfg, W, H, slots, width):
         res[name] = r
     p, f = res["pert64"], res["fexp32"]
     esc = (p["smooth"] >= 0) & (f["smooth"] >= 0)
-    out = {"view": f"deep view centre, width {width}, {W}x{H}, max_iter {cfg['max_iter']}, "
+    out = {"view": f"{label} ({cx}, {cy}), width {width}, {W}x{H}, max_iter {max_iter}, "
                    f"chained BLA d=0..15 tol 1e-6, {slots} frames in flight",
+           "escaped": float(np.mean(p["smooth"] >= 0)),
            "count_agreement": float(np.mean(p["counts"] == f["counts"])),
            "max_smooth_diff": float(np.max(np.abs(p["smooth"][esc] - f["smooth"][esc]), initial=0.0)),
+           "p99_smooth_diff": float(np.percentile(np.abs(p["smooth"][esc] - f["smooth"][esc]), 99))
+           if esc.any() else 0.0,
            "ratio": f["giters"] / p["giters"] if p["giters"] else None}
     for name, r in res.items():
         out[name] = {"value": r["giters"], "ms_per_step": r["wall_ms"], "pass_ms": r["kernel_ms"],
                      "one_at_a_time_ms": r["single_wall_ms"], "iterations": r["iters"],
                      "executed_steps": {"plain": r["plain"], "bla": r["bla"]},
                      "oracle_block_exact": r["oracle_block_exact"]}
-    log(f"vs FP64 delta at {width}: {out}")
+    log(f"vs FP64 delta, {label} at {width}: {out}")
     return out
 
 
